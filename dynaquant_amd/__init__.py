@@ -1,0 +1,279 @@
+"""dynaquant_amd -- MI355X (gfx950) stochastic exciton-dynamics engine.
+
+Python mirror of the reference's system / propagate / spectra seams over
+the C ABI in include/dynaquant.h (libdynaquant_amd.so, built in-tree by
+dynaquant_amd.build).  Loading fails loudly when the library is missing:
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdynaquant_amd.so")
+
+FP64, MIXED = 0, 1
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "BREAKDOWN",
+          5: "NONFINITE", 6: "SCALING_OVERFLOW", 7: "NO_DEVICE", 8: "STATE"}
+
+EXPORTS = [
+    "dq_system_create", "dq_system_destroy", "dq_last_error", "dq_sync",
+    "dq_system_set_hamiltonian_csr", "dq_system_set_ou_params", "dq_system_set_trajectories",
+    "dq_system_reset_disorder", "dq_system_set_state", "dq_system_get_state",
+    "dq_system_set_observables", "dq_tube_sites", "dq_tube_couplings", "dq_update_noise",
+    "dq_propagate_step", "dq_run", "dq_get_step", "dq_get_populations", "dq_get_corr",
+    "dq_get_error_sum", "dq_get_diagonal", "dq_spmv", "dq_pade_tbar", "dq_spectrum", "dq_fft",
+    "dq_last_run_ms",
+]
+
+
+class DQError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n_sites", C.c_int32), ("n_traj", C.c_int32), ("krylov_m", C.c_int32),
+        ("krylov_substeps", C.c_int32), ("ou_substeps", C.c_int32), ("precision", C.c_int32),
+        ("force_batched", C.c_int32), ("reserved", C.c_int32),
+        ("dt_fs", C.c_double), ("hbar_cm_fs", C.c_double), ("coupling_scale", C.c_double),
+        ("sigma_static", C.c_double), ("breakdown_tol", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class TubeSpec(C.Structure):
+    _fields_ = [
+        ("n_walls", C.c_int32), ("per_ring", C.c_int32), ("order", C.c_int32),
+        ("rings", C.c_int32 * 4), ("radius_nm", C.c_double * 4), ("ring_dz_nm", C.c_double),
+        ("mu_debye", C.c_double), ("tilt_deg", C.c_double), ("cutoff_nm", C.c_double),
+        ("kappa", C.c_double),
+    ]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libdynaquant_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libdynaquant_amd.so not built: run python -m dynaquant_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.dq_last_error.restype = C.c_char_p
+        for name in EXPORTS:
+            f = getattr(L, name)
+            if name != "dq_last_error":
+                f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise DQError(code, lib().dq_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def c2i(z):
+    return np.ascontiguousarray(np.asarray(z, np.complex128)).view(np.float64).copy()
+
+
+# --------------------------------------------------------------------------
+# Hamiltonian generator
+# --------------------------------------------------------------------------
+def tube_spec(rings, radius, per_ring=60, order=0, ring_dz=0.8, mu_debye=10.0, tilt_deg=30.0,
+              cutoff=2.5, kappa=5.04):
+    s = TubeSpec()
+    s.n_walls = len(rings)
+    s.per_ring = per_ring
+    s.order = order
+    for w in range(len(rings)):
+        s.rings[w] = int(rings[w])
+        s.radius_nm[w] = float(radius[w])
+    s.ring_dz_nm, s.mu_debye, s.tilt_deg, s.cutoff_nm, s.kappa = ring_dz, mu_debye, tilt_deg, cutoff, kappa
+    return s
+
+
+def tube(rings, radius, **kw):
+    """-> (pos[n,3], dip[n,3], wall[n], rowptr, col, val) of a dipole tube."""
+    s = tube_spec(rings, radius, **kw)
+    n = C.c_int32()
+    _check(lib().dq_tube_sites(C.byref(s), None, None, None, C.byref(n)))
+    n = n.value
+    pos = np.zeros(3 * n)
+    dip = np.zeros(3 * n)
+    wall = np.zeros(n, np.int32)
+    _check(lib().dq_tube_sites(C.byref(s), _p(pos), _p(dip), _p(wall), None))
+    rowptr = np.zeros(n + 1, np.int32)
+    nnz = C.c_int64()
+    _check(lib().dq_tube_couplings(C.byref(s), _p(rowptr), None, None, C.byref(nnz)))
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1))
+    _check(lib().dq_tube_couplings(C.byref(s), _p(rowptr), _p(col), _p(val), None))
+    return pos.reshape(n, 3), dip.reshape(n, 3), wall, rowptr, col[: nnz.value], val[: nnz.value]
+
+
+# --------------------------------------------------------------------------
+# System (systemClass / propagate / spectraClass seams)
+# --------------------------------------------------------------------------
+class System:
+    def __init__(self, n, m, dt_fs, n_traj=1, krylov_substeps=1, ou_substeps=1, precision=FP64,
+                 hbar=0.0, coupling_scale=1.0, sigma_static=0.0, breakdown_tol=0.0, seed=1,
+                 force_batched=False):
+        cfg = Config(n, n_traj, m, krylov_substeps, ou_substeps, precision, int(force_batched), 0,
+                     dt_fs, hbar, coupling_scale, sigma_static, breakdown_tol, seed)
+        self.cfg = cfg
+        self.n, self.B, self.m = n, n_traj, m
+        h = C.c_void_p()
+        _check(lib().dq_system_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.every = 0
+        self.ngroups = 0
+
+    def close(self):
+        if self._h:
+            lib().dq_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_hamiltonian(self, rowptr, col, val, e_site=None):
+        rowptr, col, val = _c(rowptr, np.int32), _c(col, np.int32), _c(val, np.float64)
+        es = None if e_site is None else _c(e_site, np.float64)
+        _check(lib().dq_system_set_hamiltonian_csr(self._h, self.n, _p(rowptr), _p(col), _p(val), _p(es)))
+
+    def set_ou(self, tau_fs, sigma):
+        tau = _c(np.broadcast_to(tau_fs, (self.n,)), np.float64)
+        sig = _c(np.broadcast_to(sigma, (self.n,)), np.float64)
+        _check(lib().dq_system_set_ou_params(self._h, _p(tau), _p(sig)))
+
+    def set_trajectories(self, traj_ids=None, n_active=None):
+        t = None if traj_ids is None else _c(traj_ids, np.uint64)
+        a = None if n_active is None else _c(n_active, np.int32)
+        _check(lib().dq_system_set_trajectories(self._h, _p(t), _p(a)))
+
+    def reset_disorder(self):
+        _check(lib().dq_system_reset_disorder(self._h))
+
+    def set_state(self, psi):
+        psi = np.asarray(psi, np.complex128)
+        if psi.ndim == 1:
+            psi = np.broadcast_to(psi, (self.B, self.n))
+        a = c2i(psi.reshape(-1))
+        _check(lib().dq_system_set_state(self._h, _p(a)))
+
+    def get_state(self):
+        a = np.zeros(2 * self.n * self.B)
+        _check(lib().dq_system_get_state(self._h, _p(a)))
+        return a.view(np.complex128).reshape(self.B, self.n)
+
+    def set_observables(self, group=None, ngroups=0, every=0, max_steps=0, bra=None):
+        g = None if group is None else _c(group, np.int32)
+        br = None if bra is None else c2i(bra)
+        _check(lib().dq_system_set_observables(self._h, _p(g), ngroups, every, max_steps, _p(br)))
+        self.every, self.ngroups = every, ngroups
+
+    def update_noise(self):
+        _check(lib().dq_update_noise(self._h))
+
+    def step(self):
+        _check(lib().dq_propagate_step(self._h))
+
+    def run(self, nsteps, graph=False):
+        _check(lib().dq_run(self._h, C.c_int64(nsteps), int(graph)))
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        _check(lib().dq_last_run_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def get_step(self):
+        s = C.c_int64()
+        _check(lib().dq_get_step(self._h, C.byref(s)))
+        return s.value
+
+    def populations(self):
+        nrec = C.c_int64()
+        _check(lib().dq_get_populations(self._h, None, C.byref(nrec)))
+        G = self.ngroups
+        out = np.zeros(self.B * nrec.value * G)
+        _check(lib().dq_get_populations(self._h, _p(out), C.byref(nrec)))
+        return out.reshape(self.B, nrec.value, G)
+
+    def corr(self):
+        T = self.get_step()
+        out = np.zeros(2 * self.B * (T + 1))
+        _check(lib().dq_get_corr(self._h, _p(out)))
+        return out.view(np.complex128).reshape(self.B, T + 1)
+
+    def error_sum(self):
+        out = np.zeros(self.B)
+        _check(lib().dq_get_error_sum(self._h, _p(out)))
+        return out
+
+    def diagonal(self):
+        out = np.zeros(self.B * self.n)
+        _check(lib().dq_get_diagonal(self._h, _p(out)))
+        return out.reshape(self.B, self.n)
+
+    def spmv(self, traj, x):
+        xi = c2i(x)
+        y = np.zeros(2 * self.n)
+        _check(lib().dq_spmv(self._h, traj, _p(xi), _p(y)))
+        return y.view(np.complex128)
+
+    def spectrum(self, sigma_fs, mu2, nfft):
+        om = np.zeros(nfft)
+        sp = np.zeros(2 * nfft)
+        _check(lib().dq_spectrum(self._h, C.c_double(sigma_fs), C.c_double(mu2), nfft, _p(om), _p(sp)))
+        return om, sp.view(np.complex128)
+
+
+def pade_tbar(m, alpha, beta, theta):
+    """exp(-i theta Tbar) on the GPU for each row of alpha/beta ([count, m])."""
+    alpha = _c(np.atleast_2d(alpha), np.float64)
+    beta = _c(np.atleast_2d(beta), np.float64)
+    count = alpha.shape[0]
+    n = m + 2
+    E = np.zeros(count * n * n * 2)
+    s = np.zeros(count, np.int32)
+    _check(lib().dq_pade_tbar(m, count, _p(alpha), _p(beta), C.c_double(theta), _p(E), _p(s)))
+    E = E.view(np.complex128).reshape(count, n, n).transpose(0, 2, 1)  # column-major -> [row, col]
+    return E, s
+
+
+def fft(x, sign=+1):
+    """unnormalised FFT on the GPU of each row of x (length a power of two)"""
+    x = np.atleast_2d(np.asarray(x, np.complex128))
+    a = c2i(x.reshape(-1))
+    _check(lib().dq_fft(x.shape[1], x.shape[0], _p(a), sign))
+    return a.view(np.complex128).reshape(x.shape)
+
+
+def hbar_ref():
+    """hbar in cm^-1 fs as the reference computes it (cppUtility.cpp:10-35)"""
+    pi = np.arctan(1.0) * 4
+    return (1 / 100.) * (1e15) * 1. / (299792548. * 2 * pi)

@@ -1,0 +1,724 @@
+// dq_api.cpp -- host side of the C ABI (include/dynaquant.h).
+//
+// Owns all device memory of a system, drives the per-step kernel sequence
+// on one HIP stream (optionally replayed from a captured hipGraph) and
+// converts between the caller's [b][i] interleaved-complex buffers and the
+// engine's [i][b] device layout.  The reference keeps this orchestration in
+// propagate.cu:167-298 and systemClass.cu, with a host sync after every
+// SpMV; here the host only enqueues.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dynaquant.h"
+#include "dq_internal.h"
+
+using dq::Dev;
+
+namespace {
+
+thread_local std::string g_err;
+
+dq_status fail(dq_status st, const std::string& msg)
+{
+    g_err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? DQ_ERR_OOM : DQ_ERR_HIP,            \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+#define DQ_TRY(expr)                       \
+    do {                                   \
+        dq_status s_ = (expr);             \
+        if (s_ != DQ_OK) return s_;        \
+    } while (0)
+
+double reference_hbar()
+{
+    // cppUtility.cpp:19-22, same evaluation order
+    double pi = std::atan(1) * 4;
+    double lightspeed = 299792548.;
+    return (1 / 100.) * (1e15) * 1. / (lightspeed * 2 * pi);
+}
+
+}  // namespace
+
+struct dq_system {
+    dq_config cfg{};
+    Dev d{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph = nullptr;
+    double last_ms = 0.0;
+    int n = 0, B = 0, Bp = 0, m = 0, K = 1;
+    bool have_h = false, have_ou = false, have_state = false, have_disorder = false;
+    int64_t step = 0;                 // host mirror of the device counter
+    int64_t obs_cap_rec = 0;          // records allocated
+    std::vector<void*> allocs;        // everything owned
+    // host copies
+    std::vector<int32_t> rowptr, col;
+    std::vector<double> val;
+    // observable config
+    int ngroups = 0, every = 0, site_mode = 0;
+    int64_t max_steps = 0;
+    bool have_bra = false;
+
+    ~dq_system()
+    {
+        if (graph) hipGraphExecDestroy(graph);
+        for (void* p : allocs) hipFree(p);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    template <typename T>
+    dq_status alloc(T** p, size_t count)
+    {
+        void* q = nullptr;
+        const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+        if (getenv("DQ_TEST_FAIL_ALLOC")) return fail(DQ_ERR_OOM, "allocation refused (DQ_TEST_FAIL_ALLOC)");
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(DQ_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        hipMemsetAsync(q, 0, bytes, stream);
+        *p = static_cast<T*>(q);
+        return DQ_OK;
+    }
+    template <typename T>
+    void release(T*& p)
+    {
+        if (!p) return;
+        for (auto& q : allocs)
+            if (q == (void*)p) { hipFree(q); q = nullptr; }
+        p = nullptr;
+    }
+    void drop_graph()
+    {
+        if (graph) { hipGraphExecDestroy(graph); graph = nullptr; }
+    }
+};
+
+extern "C" {
+
+const char* dq_last_error(void) { return g_err.c_str(); }
+
+dq_status dq_system_create(const dq_config* cfg, dq_system** out)
+{
+    if (!cfg || !out) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->n_sites <= 0 || cfg->n_traj <= 0) return fail(DQ_ERR_INVALID_ARG, "n_sites and n_traj must be > 0");
+    if (cfg->krylov_m < 2 || cfg->krylov_m > dq::kMaxM) return fail(DQ_ERR_INVALID_ARG, "krylov_m must be in [2, 32]");
+    if (cfg->krylov_substeps < 1) return fail(DQ_ERR_INVALID_ARG, "krylov_substeps must be >= 1");
+    if (cfg->ou_substeps < 1) return fail(DQ_ERR_INVALID_ARG, "ou_substeps must be >= 1");
+    if (cfg->precision != DQ_PREC_FP64 && cfg->precision != DQ_PREC_MIXED)
+        return fail(DQ_ERR_INVALID_ARG, "unknown precision");
+    if (!(cfg->dt_fs > 0.0)) return fail(DQ_ERR_INVALID_ARG, "dt_fs must be > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(DQ_ERR_NO_DEVICE, "no HIP device");
+    dq_system* s = new dq_system();
+    s->cfg = *cfg;
+    s->n = cfg->n_sites;
+    s->B = cfg->n_traj;
+    s->m = cfg->krylov_m;
+    s->K = cfg->krylov_substeps;
+    const bool plain = (s->B == 1 && !cfg->force_batched && cfg->precision == DQ_PREC_FP64);
+    s->Bp = plain ? 1 : ((s->B + 63) / 64) * 64;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete s; return fail(DQ_ERR_HIP, std::string("stream: ") + hipGetErrorString(e)); }
+    hipEventCreate(&s->ev0);
+    hipEventCreate(&s->ev1);
+    Dev& d = s->d;
+    d.n = s->n; d.Bp = s->Bp; d.B = s->B; d.m = s->m;
+    d.mixed = cfg->precision == DQ_PREC_MIXED;
+    d.plain = plain;
+    d.rt_e = plain ? 1024 : 128;
+    d.rt_s = plain ? 64 : 32;
+    d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
+    d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
+    d.S = cfg->ou_substeps;
+    d.seed = cfg->seed;
+    // systemClass.cu:993-994: subdt = dt / S, sqrt(subdt)
+    d.subdt = cfg->dt_fs / (double)cfg->ou_substeps;
+    d.sqrt_subdt = std::sqrt(d.subdt);
+    d.g = cfg->coupling_scale;
+    const double hbar = cfg->hbar_cm_fs > 0.0 ? cfg->hbar_cm_fs : reference_hbar();
+    // propagate.cu:90-92: expCoeff = -i (1/hbar) dt, here per Krylov substep
+    d.theta = -((-1. / hbar) * (cfg->dt_fs / (double)cfg->krylov_substeps));
+    d.tol = cfg->breakdown_tol > 0.0 ? cfg->breakdown_tol : (d.mixed ? 1e-5 : 1e-10);
+    d.sigma_static = cfg->sigma_static;
+    const size_t nB = (size_t)d.n * d.Bp;
+    dq_status st = DQ_OK;
+    double2* psi = nullptr;
+    void* V = nullptr;
+    void* diag = nullptr;
+#define ALLOC(p, cnt) do { if ((st = s->alloc(p, cnt)) != DQ_OK) { delete s; return st; } } while (0)
+    ALLOC(&psi, nB);
+    if (d.mixed) { float2* v; ALLOC(&v, nB * (s->m + 1)); V = v; float* dg; ALLOC(&dg, nB); diag = dg; }
+    else { double2* v; ALLOC(&v, nB * (s->m + 1)); V = v; double* dg; ALLOC(&dg, nB); diag = dg; }
+    d.psi = psi; d.V = V; d.diag = diag;
+    ALLOC(&d.de, nB);
+    ALLOC(&d.e0, nB);
+    ALLOC(&d.sc, (size_t)(s->m + 1) * d.Bp);
+    ALLOC(&d.alpha, (size_t)s->m * d.Bp);
+    ALLOC(&d.beta, (size_t)s->m * d.Bp);
+    ALLOC(&d.hco, (size_t)dq::kMaxM * dq::kMaxM * d.Bp);
+    ALLOC(&d.gram, (size_t)(s->m + 1) * d.Bp);
+    ALLOC(&d.nrm2, (size_t)d.Bp);
+    ALLOC(&d.ycoef, (size_t)s->m * d.Bp);
+    ALLOC(&d.err_sum, (size_t)d.Bp);
+    ALLOC(&d.broke, (size_t)d.Bp);
+    ALLOC(&d.status, 1);
+    size_t nq = std::max<size_t>(2 * (s->m + 1) + 1, dq::kMaxGroups + 2);
+    size_t part = std::max(nq * d.ntile_e, (size_t)2 * d.ntile_s) * d.Bp;
+    ALLOC(&d.part, part);
+    ALLOC(&d.red, nq * d.Bp);
+    ALLOC(&d.step, 1);
+    // trajectories: ids 0..B-1, all sites active; padding trajectories inactive
+    {
+        int32_t* na; uint64_t* ti;
+        ALLOC(&na, (size_t)d.Bp);
+        ALLOC(&ti, (size_t)d.Bp);
+        std::vector<int32_t> hn(d.Bp, 0);
+        std::vector<uint64_t> ht(d.Bp, 0);
+        for (int b = 0; b < s->B; ++b) { hn[b] = s->n; ht[b] = (uint64_t)b; }
+        hipMemcpyAsync(na, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
+        hipMemcpyAsync(ti, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
+        d.nact = na; d.traj_id = ti;
+    }
+#undef ALLOC
+    dq::launch_set_unit_norm(d, s->stream);
+    e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) { delete s; return fail(DQ_ERR_HIP, std::string("create: ") + hipGetErrorString(e)); }
+    *out = s;
+    return DQ_OK;
+}
+
+dq_status dq_system_destroy(dq_system* sys)
+{
+    if (!sys) return DQ_OK;
+    hipStreamSynchronize(sys->stream);
+    delete sys;
+    return DQ_OK;
+}
+
+dq_status dq_sync(dq_system* sys)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    int flags = 0;
+    HIP_TRY(hipMemcpy(&flags, sys->d.status, sizeof(int), hipMemcpyDeviceToHost));
+    if (flags & dq::kStatusScaling) return fail(DQ_ERR_SCALING_OVERFLOW, "Pade scaling exponent s >= 50");
+    if (flags & dq::kStatusNonfinite) return fail(DQ_ERR_NONFINITE, "non-finite error estimate");
+    return DQ_OK;
+}
+
+dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t* rowptr,
+                                        const int32_t* col, const double* val, const double* e_site)
+{
+    if (!sys || !rowptr) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (n != sys->n) return fail(DQ_ERR_INVALID_ARG, "n does not match n_sites");
+    if (rowptr[0] != 0) return fail(DQ_ERR_INVALID_ARG, "rowptr[0] must be 0");
+    const int64_t nnz = rowptr[n];
+    for (int i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i]) return fail(DQ_ERR_INVALID_ARG, "rowptr not monotone");
+    if (nnz > 0 && (!col || !val)) return fail(DQ_ERR_INVALID_ARG, "null col/val");
+    for (int64_t p = 0; p < nnz; ++p) {
+        if (col[p] < 0 || col[p] >= n) return fail(DQ_ERR_INVALID_ARG, "column index out of range");
+        if (!std::isfinite(val[p])) return fail(DQ_ERR_NONFINITE, "non-finite coupling");
+    }
+    for (int i = 0; i < n; ++i)
+        for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p)
+            if (col[p] == i) return fail(DQ_ERR_INVALID_ARG, "diagonal entries belong in e_site");
+    Dev& d = sys->d;
+    sys->drop_graph();
+    sys->rowptr.assign(rowptr, rowptr + n + 1);
+    sys->col.assign(col, col + nnz);
+    sys->val.assign(val, val + nnz);
+    int32_t *rp, *cl;
+    double *v64, *es;
+    float* v32;
+    hipStreamSynchronize(sys->stream);
+    sys->release(*(int32_t**)&d.rowptr);
+    sys->release(*(int32_t**)&d.col);
+    sys->release(*(double**)&d.val64);
+    sys->release(*(float**)&d.val32);
+    sys->release(*(double**)&d.e_site);
+    DQ_TRY(sys->alloc(&rp, (size_t)n + 1));
+    DQ_TRY(sys->alloc(&cl, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(sys->alloc(&v64, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(sys->alloc(&v32, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(sys->alloc(&es, (size_t)n));
+    std::vector<float> f32(nnz);
+    for (int64_t p = 0; p < nnz; ++p) f32[p] = (float)val[p];
+    std::vector<double> esv(n, 0.0);
+    if (e_site) for (int i = 0; i < n; ++i) esv[i] = e_site[i];
+    HIP_TRY(hipMemcpyAsync(rp, rowptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, sys->stream));
+    if (nnz > 0) {
+        HIP_TRY(hipMemcpyAsync(cl, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(v64, val, sizeof(double) * nnz, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(v32, f32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, sys->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(es, esv.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.rowptr = rp; d.col = cl; d.val64 = v64; d.val32 = v32; d.e_site = es;
+    sys->have_h = true;
+    return DQ_OK;
+}
+
+dq_status dq_system_set_ou_params(dq_system* sys, const double* tau_fs, const double* sigma)
+{
+    if (!sys || !tau_fs || !sigma) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    const int n = sys->n;
+    std::vector<double> it(n), co(n), sg(n);
+    for (int i = 0; i < n; ++i) {
+        if (!(tau_fs[i] > 0.0) || !(sigma[i] >= 0.0)) return fail(DQ_ERR_INVALID_ARG, "tau must be > 0, sigma >= 0");
+        // systemClass.cu:1033-1034
+        it[i] = 1. / tau_fs[i];
+        co[i] = sigma[i] * std::sqrt(2. / tau_fs[i]);
+        sg[i] = sigma[i];
+        if (sigma[i] == 0.0) { it[i] = 0.0; co[i] = 0.0; }
+    }
+    Dev& d = sys->d;
+    sys->drop_graph();
+    hipStreamSynchronize(sys->stream);
+    sys->release(*(double**)&d.inv_tau);
+    sys->release(*(double**)&d.coeff);
+    sys->release(*(double**)&d.ou_sigma);
+    double *a, *b, *c;
+    DQ_TRY(sys->alloc(&a, n));
+    DQ_TRY(sys->alloc(&b, n));
+    DQ_TRY(sys->alloc(&c, n));
+    HIP_TRY(hipMemcpyAsync(a, it.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(b, co.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(c, sg.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.inv_tau = a; d.coeff = b; d.ou_sigma = c;
+    sys->have_ou = true;
+    return DQ_OK;
+}
+
+dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, const int32_t* n_active)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    Dev& d = sys->d;
+    std::vector<int32_t> hn(d.Bp, 0);
+    std::vector<uint64_t> ht(d.Bp, 0);
+    for (int b = 0; b < sys->B; ++b) {
+        ht[b] = traj_ids ? traj_ids[b] : (uint64_t)b;
+        hn[b] = n_active ? n_active[b] : sys->n;
+        if (hn[b] < 1 || hn[b] > sys->n) return fail(DQ_ERR_INVALID_ARG, "n_active out of range");
+    }
+    HIP_TRY(hipMemcpyAsync((void*)d.nact, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync((void*)d.traj_id, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->have_disorder = false;
+    return DQ_OK;
+}
+
+dq_status dq_system_reset_disorder(dq_system* sys)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    if (!sys->have_h || !sys->have_ou) return fail(DQ_ERR_STATE, "set the Hamiltonian and OU parameters first");
+    Dev& d = sys->d;
+    HIP_TRY(dq::launch_init_disorder(d, sys->stream));
+    HIP_TRY(hipMemsetAsync(d.err_sum, 0, sizeof(double) * d.Bp, sys->stream));
+    HIP_TRY(hipMemsetAsync(d.step, 0, sizeof(int64_t), sys->stream));
+    HIP_TRY(hipMemsetAsync(d.status, 0, sizeof(int), sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->step = 0;
+    sys->have_disorder = true;
+    return DQ_OK;
+}
+
+dq_status dq_system_set_state(dq_system* sys, const double* psi)
+{
+    if (!sys || !psi) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    Dev& d = sys->d;
+    const int n = sys->n, Bp = d.Bp;
+    std::vector<double> h((size_t)2 * n * Bp, 0.0);
+    for (int b = 0; b < sys->B; ++b)
+        for (int i = 0; i < n; ++i) {
+            const double re = psi[((size_t)b * n + i) * 2], im = psi[((size_t)b * n + i) * 2 + 1];
+            if (!std::isfinite(re) || !std::isfinite(im)) return fail(DQ_ERR_NONFINITE, "non-finite state");
+            h[((size_t)i * Bp + b) * 2] = re;
+            h[((size_t)i * Bp + b) * 2 + 1] = im;
+        }
+    HIP_TRY(hipMemcpyAsync(d.psi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(dq::launch_set_unit_norm(d, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->have_state = true;
+    return DQ_OK;
+}
+
+dq_status dq_system_get_state(dq_system* sys, double* psi)
+{
+    if (!sys || !psi) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    Dev& d = sys->d;
+    const int n = sys->n, Bp = d.Bp;
+    std::vector<double> h((size_t)2 * n * Bp);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(h.data(), d.psi, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b)
+        for (int i = 0; i < n; ++i) {
+            psi[((size_t)b * n + i) * 2] = h[((size_t)i * Bp + b) * 2];
+            psi[((size_t)b * n + i) * 2 + 1] = h[((size_t)i * Bp + b) * 2 + 1];
+        }
+    return DQ_OK;
+}
+
+dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_t ngroups, int32_t every,
+                                    int64_t max_steps, const double* bra)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    Dev& d = sys->d;
+    const int n = sys->n;
+    if (max_steps < 0 || every < 0) return fail(DQ_ERR_INVALID_ARG, "negative every/max_steps");
+    if (every > 0 && (!group || ngroups <= 0)) return fail(DQ_ERR_INVALID_ARG, "groups required");
+    int site_mode = 0;
+    if (every > 0) {
+        site_mode = (ngroups == n);
+        for (int i = 0; i < n && site_mode; ++i) if (group[i] != i) site_mode = 0;
+        if (!site_mode && ngroups > dq::kMaxGroups)
+            return fail(DQ_ERR_INVALID_ARG, "at most 8 population groups (or one per site)");
+        for (int i = 0; i < n; ++i)
+            if (group[i] >= ngroups) return fail(DQ_ERR_INVALID_ARG, "group index out of range");
+    }
+    sys->drop_graph();
+    hipStreamSynchronize(sys->stream);
+    sys->release(*(int32_t**)&d.group);
+    sys->release(d.pops);
+    sys->release(*(double2**)&d.bra);
+    sys->release(d.corr);
+    const int64_t nrec = every > 0 ? max_steps / every + 1 : 0;
+    if (every > 0) {
+        int32_t* gp;
+        DQ_TRY(sys->alloc(&gp, n));
+        HIP_TRY(hipMemcpyAsync(gp, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, sys->stream));
+        d.group = gp;
+        DQ_TRY(sys->alloc(&d.pops, (size_t)nrec * (site_mode ? n : ngroups) * d.Bp));
+    }
+    if (bra) {
+        double2* br;
+        DQ_TRY(sys->alloc(&br, n));
+        HIP_TRY(hipMemcpyAsync(br, bra, sizeof(double2) * n, hipMemcpyHostToDevice, sys->stream));
+        d.bra = br;
+        DQ_TRY(sys->alloc(&d.corr, (size_t)(max_steps + 1) * d.Bp));
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.ngroups = ngroups;
+    d.every = every;
+    d.site_mode = site_mode;
+    d.max_steps = max_steps;
+    sys->ngroups = ngroups;
+    sys->every = every;
+    sys->site_mode = site_mode;
+    sys->max_steps = max_steps;
+    sys->obs_cap_rec = nrec;
+    sys->have_bra = bra != nullptr;
+    return DQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step sequence
+// ---------------------------------------------------------------------------
+static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
+{
+    const Dev& d = sys->d;
+    hipStream_t st = sys->stream;
+    hipError_t e;
+    const bool obs = (d.every > 0 || d.bra != nullptr);
+    if ((e = dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st)) != hipSuccess) return e;
+    if (first_in_step && obs && (e = dq::launch_reduce_obs(d, st)) != hipSuccess) return e;
+    for (int j = 0; j < d.m; ++j) {
+        if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
+        if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
+        if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
+        if ((e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;
+    }
+    if ((e = dq::launch_spmm(d, d.m, 1, st)) != hipSuccess) return e;
+    if ((e = dq::launch_pade(d, st)) != hipSuccess) return e;
+    if ((e = dq::launch_combine(d, st)) != hipSuccess) return e;
+    return dq::launch_finish_norm(d, last_in_step ? 1 : 0, st);
+}
+
+static hipError_t enqueue_step(dq_system* sys)
+{
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < sys->K; ++k)
+        if ((e = enqueue_cycle(sys, k == 0, k == sys->K - 1)) != hipSuccess) return e;
+    return e;
+}
+
+static hipError_t enqueue_finalize(dq_system* sys)
+{
+    const Dev& d = sys->d;
+    const bool obs = (d.every > 0 || d.bra != nullptr);
+    hipError_t e = dq::launch_prologue(d, 1, 0, obs, 0, sys->stream);
+    if (e != hipSuccess) return e;
+    if (obs && (e = dq::launch_reduce_obs(d, sys->stream)) != hipSuccess) return e;
+    return dq::launch_set_unit_norm(d, sys->stream);
+}
+
+static dq_status check_ready(dq_system* sys)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    if (!sys->have_h || !sys->have_ou) return fail(DQ_ERR_STATE, "Hamiltonian/OU parameters not set");
+    if (!sys->have_disorder) return fail(DQ_ERR_STATE, "call dq_system_reset_disorder first");
+    if (!sys->have_state) return fail(DQ_ERR_STATE, "call dq_system_set_state first");
+    return DQ_OK;
+}
+
+dq_status dq_update_noise(dq_system* sys)
+{
+    DQ_TRY(check_ready(sys));
+    HIP_TRY(dq::launch_prologue(sys->d, 0, 1, 0, 0, sys->stream));
+    HIP_TRY(dq::launch_advance_step(sys->d, sys->stream));
+    sys->step += 1;
+    return dq_sync(sys);
+}
+
+dq_status dq_propagate_step(dq_system* sys)
+{
+    return dq_run(sys, 1, 0);
+}
+
+dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
+{
+    DQ_TRY(check_ready(sys));
+    if (nsteps < 0) return fail(DQ_ERR_INVALID_ARG, "nsteps < 0");
+    if ((sys->every > 0 || sys->have_bra) && sys->step + nsteps > sys->max_steps)
+        return fail(DQ_ERR_INVALID_ARG, "run exceeds max_steps of the observables");
+    hipStream_t st = sys->stream;
+    if (use_graph && nsteps > 0 && !sys->graph) {
+        hipGraph_t g;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_step(sys);
+        hipError_t e2 = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess || e2 != hipSuccess)
+            return fail(DQ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        e = hipGraphInstantiate(&sys->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) { sys->graph = nullptr; return fail(DQ_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+    }
+    HIP_TRY(hipEventRecord(sys->ev0, st));
+    for (int64_t k = 0; k < nsteps; ++k) {
+        if (use_graph) HIP_TRY(hipGraphLaunch(sys->graph, st));
+        else HIP_TRY(enqueue_step(sys));
+    }
+    HIP_TRY(hipEventRecord(sys->ev1, st));
+    HIP_TRY(enqueue_finalize(sys));
+    HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+    sys->last_ms = ms;
+    sys->step += nsteps;
+    return dq_sync(sys);
+}
+
+dq_status dq_get_step(dq_system* sys, int64_t* step)
+{
+    if (!sys || !step) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    int64_t s = 0;
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(&s, sys->d.step, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *step = s;
+    return DQ_OK;
+}
+
+dq_status dq_last_run_ms(dq_system* sys, double* ms)
+{
+    if (!sys || !ms) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    *ms = sys->last_ms;
+    return DQ_OK;
+}
+
+dq_status dq_get_populations(dq_system* sys, double* pops, int64_t* nrec_out)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    if (sys->every <= 0) return fail(DQ_ERR_STATE, "no population observables set");
+    const Dev& d = sys->d;
+    int64_t nrec = std::min<int64_t>(sys->step / sys->every + 1, sys->obs_cap_rec);
+    if (nrec_out) *nrec_out = nrec;
+    if (!pops) return DQ_OK;
+    const int G = sys->site_mode ? sys->n : sys->ngroups;
+    std::vector<double> h((size_t)nrec * G * d.Bp);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(h.data(), d.pops, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b)
+        for (int64_t r = 0; r < nrec; ++r)
+            for (int g = 0; g < G; ++g)
+                pops[((size_t)b * nrec + r) * G + g] = h[((size_t)r * G + g) * d.Bp + b];
+    return DQ_OK;
+}
+
+dq_status dq_get_corr(dq_system* sys, double* corr)
+{
+    if (!sys || !corr) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (!sys->have_bra) return fail(DQ_ERR_STATE, "no bra set");
+    const Dev& d = sys->d;
+    const int64_t T = std::min(sys->step, sys->max_steps);
+    std::vector<double2> h((size_t)(T + 1) * d.Bp);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(h.data(), d.corr, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b)
+        for (int64_t k = 0; k <= T; ++k) {
+            corr[((size_t)b * (T + 1) + k) * 2] = h[(size_t)k * d.Bp + b].x;
+            corr[((size_t)b * (T + 1) + k) * 2 + 1] = h[(size_t)k * d.Bp + b].y;
+        }
+    return DQ_OK;
+}
+
+dq_status dq_get_error_sum(dq_system* sys, double* err)
+{
+    if (!sys || !err) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    std::vector<double> h(sys->d.Bp);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(h.data(), sys->d.err_sum, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b) err[b] = h[b];
+    return DQ_OK;
+}
+
+dq_status dq_get_diagonal(dq_system* sys, double* diag)
+{
+    if (!sys || !diag) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    const Dev& d = sys->d;
+    const size_t nB = (size_t)sys->n * d.Bp;
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    if (d.mixed) {
+        std::vector<float> h(nB);
+        HIP_TRY(hipMemcpy(h.data(), d.diag, sizeof(float) * nB, hipMemcpyDeviceToHost));
+        for (int b = 0; b < sys->B; ++b)
+            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + b];
+    } else {
+        std::vector<double> h(nB);
+        HIP_TRY(hipMemcpy(h.data(), d.diag, sizeof(double) * nB, hipMemcpyDeviceToHost));
+        for (int b = 0; b < sys->B; ++b)
+            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + b];
+    }
+    return DQ_OK;
+}
+
+dq_status dq_spmv(dq_system* sys, int32_t traj, const double* x, double* y)
+{
+    if (!sys || !x || !y) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (!sys->have_h || !sys->have_disorder) return fail(DQ_ERR_STATE, "Hamiltonian and disorder required");
+    if (traj < 0 || traj >= sys->B) return fail(DQ_ERR_INVALID_ARG, "trajectory out of range");
+    const Dev& d = sys->d;
+    const int n = sys->n, Bp = d.Bp;
+    const size_t nB = (size_t)n * Bp;
+    if (d.mixed) {
+        std::vector<float2> h(nB, make_float2(0.f, 0.f));
+        for (int i = 0; i < n; ++i) h[(size_t)i * Bp + traj] = make_float2((float)x[2 * i], (float)x[2 * i + 1]);
+        HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(float2) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        HIP_TRY(hipMemcpy(h.data(), (float2*)d.V + nB, sizeof(float2) * nB, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) { y[2 * i] = h[(size_t)i * Bp + traj].x; y[2 * i + 1] = h[(size_t)i * Bp + traj].y; }
+    } else {
+        std::vector<double2> h(nB, make_double2(0.0, 0.0));
+        for (int i = 0; i < n; ++i) h[(size_t)i * Bp + traj] = make_double2(x[2 * i], x[2 * i + 1]);
+        HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        HIP_TRY(hipMemcpy(h.data(), (double2*)d.V + nB, sizeof(double2) * nB, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) { y[2 * i] = h[(size_t)i * Bp + traj].x; y[2 * i + 1] = h[(size_t)i * Bp + traj].y; }
+    }
+    return DQ_OK;
+}
+
+dq_status dq_pade_tbar(int32_t m, int32_t count, const double* alpha, const double* beta, double theta,
+                       double* E, int32_t* s)
+{
+    if (m < 2 || m > dq::kMaxM || count <= 0 || !alpha || !beta || !E || !s)
+        return fail(DQ_ERR_INVALID_ARG, "bad arguments");
+    const int n = m + 2;
+    double *da = nullptr, *db = nullptr;
+    double2* dE = nullptr;
+    int* ds = nullptr;
+    hipError_t e = hipSuccess;
+    auto cleanup = [&]() { hipFree(da); hipFree(db); hipFree(dE); hipFree(ds); };
+    if ((e = hipMalloc(&da, sizeof(double) * m * count)) != hipSuccess ||
+        (e = hipMalloc(&db, sizeof(double) * m * count)) != hipSuccess ||
+        (e = hipMalloc(&dE, sizeof(double2) * n * n * count)) != hipSuccess ||
+        (e = hipMalloc(&ds, sizeof(int) * count)) != hipSuccess) {
+        cleanup();
+        return fail(DQ_ERR_OOM, hipGetErrorString(e));
+    }
+    hipMemcpy(da, alpha, sizeof(double) * m * count, hipMemcpyHostToDevice);
+    hipMemcpy(db, beta, sizeof(double) * m * count, hipMemcpyHostToDevice);
+    hipMemset(dE, 0, sizeof(double2) * n * n * count);
+    e = dq::launch_pade_tbar(m, count, da, db, theta, dE, ds, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(E, dE, sizeof(double2) * n * n * count, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(s, ds, sizeof(int) * count, hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(DQ_ERR_HIP, hipGetErrorString(e));
+    for (int c = 0; c < count; ++c)
+        if (s[c] < 0) return fail(DQ_ERR_SCALING_OVERFLOW, "Pade scaling exponent s >= 50");
+    return DQ_OK;
+}
+
+dq_status dq_fft(int32_t n, int32_t batch, double* data, int32_t sign)
+{
+    if (n < 1 || (n & (n - 1)) || n > 65536 || batch < 1 || !data) return fail(DQ_ERR_INVALID_ARG, "bad FFT arguments");
+    double2 *a = nullptr, *b = nullptr;
+    const size_t bytes = sizeof(double2) * (size_t)n * batch;
+    hipError_t e;
+    if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess) {
+        hipFree(a); hipFree(b);
+        return fail(DQ_ERR_OOM, hipGetErrorString(e));
+    }
+    e = hipMemcpy(a, data, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = dq::launch_fft(n, batch, a, b, sign, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(data, a, bytes, hipMemcpyDeviceToHost);
+    hipFree(a); hipFree(b);
+    if (e != hipSuccess) return fail(DQ_ERR_HIP, hipGetErrorString(e));
+    return DQ_OK;
+}
+
+dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft, double* omega, double* spec)
+{
+    if (!sys || !omega || !spec) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (!sys->have_bra) return fail(DQ_ERR_STATE, "no correlation function (bra not set)");
+    if (nfft < 2 || (nfft & (nfft - 1))) return fail(DQ_ERR_INVALID_ARG, "nfft must be a power of two");
+    const int64_t T = std::min(sys->step, sys->max_steps);
+    if (T < 1 || T + 1 > nfft) return fail(DQ_ERR_INVALID_ARG, "need 1 <= steps < nfft");
+    double2 *a = nullptr, *b = nullptr;
+    const size_t bytes = sizeof(double2) * (size_t)nfft;
+    hipError_t e;
+    if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess) {
+        hipFree(a); hipFree(b);
+        return fail(DQ_ERR_OOM, hipGetErrorString(e));
+    }
+    e = dq::launch_spectrum_prep(sys->d, T, sys->cfg.dt_fs, sigma_fs, nfft, a, sys->stream);
+    if (e == hipSuccess) e = dq::launch_fft(nfft, 1, a, b, +1, sys->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
+    if (e == hipSuccess) e = hipMemcpy(spec, a, bytes, hipMemcpyDeviceToHost);
+    hipFree(a); hipFree(b);
+    if (e != hipSuccess) return fail(DQ_ERR_HIP, hipGetErrorString(e));
+    // spectraClass.cu:147-151 (true lightspeed on the frequency axis) and
+    // the |mu|^2 / T scaling of spectraClass.cu:641-686
+    const double inv_fs_to_inv_cm = 1. / (299792458. * 100. * 1.e-15);
+    const double scale = mu2 / (double)T;
+    for (int k = 0; k < nfft; ++k) {
+        const int kk = k > nfft / 2 ? k - nfft : k;
+        omega[k] = inv_fs_to_inv_cm * (double)kk / (sys->cfg.dt_fs * (double)nfft);
+        spec[2 * k] *= scale;
+        spec[2 * k + 1] *= scale;
+    }
+    return DQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,93 @@
+// dq_internal.h -- device state shared by the host driver (dq_api.cpp) and
+// the kernels (dq_kernels.hip).  Layouts in HBM:
+//   per-(site, trajectory) arrays are [i][b] with b contiguous (Bp = padded B),
+//   so one CSR nonzero gathers a contiguous run of trajectories;
+//   Krylov basis slot k (k = 0..m) is one such [n][Bp] array;
+//   per-trajectory scalars are [k][Bp];
+//   partial sums are [q][tile][Bp].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dq {
+
+constexpr int kMaxM = 32;          // Krylov depth limit (Pade matrix <= 34)
+constexpr int kMaxGroups = 8;      // population groups reduced on the device
+constexpr int kThreads = 256;
+
+struct Dev {
+    int n, Bp, B, m;
+    int mixed;                      // 1: float2 basis and FP32 SpMM
+    int plain;                      // 1: B == 1 single-trajectory kernels
+    int rt_e, ntile_e;              // elementwise kernels: rows per tile, tiles
+    int rt_s, ntile_s;              // SpMM kernels
+    int S;                          // OU substeps
+    uint64_t seed;
+    double subdt, sqrt_subdt;
+    double g;                       // coupling scale
+    double theta;                   // tau / hbar  (A = -i theta Tbar)
+    double tol;                     // breakdown tolerance
+    // Hamiltonian
+    const int32_t* rowptr;
+    const int32_t* col;
+    const double* val64;
+    const float* val32;
+    const double* inv_tau;          // [n]
+    const double* coeff;            // [n] sigma*sqrt(2/tau)
+    const double* ou_sigma;         // [n]
+    const double* e_site;           // [n]
+    double sigma_static;
+    // trajectories
+    const int32_t* nact;            // [Bp]
+    const uint64_t* traj_id;        // [Bp]
+    // state
+    double2* psi;                   // [n][Bp]
+    void* V;                        // (m+1) x [n][Bp] of float2 / double2
+    void* diag;                     // [n][Bp] float / double
+    double* de;                     // [n][Bp] OU offsets
+    double* e0;                     // [n][Bp] static site energies
+    // scalars
+    double* sc;                     // [m+1][Bp] scale of basis slot k
+    double* alpha;                  // [m][Bp]
+    double* beta;                   // [m][Bp]
+    double2* hco;                   // [m][m][Bp] projection coefficients h^(j)_k
+    double2* gram;                  // [m+1][Bp] Gram row of the newest slot
+    double* nrm2;                   // [Bp] |psi|^2 after combine
+    double2* ycoef;                 // [m][Bp] y_k * s_k
+    double* err_sum;                // [Bp]
+    int* broke;                     // [Bp]
+    int* status;                    // [1] sticky error flags
+    double* part;                   // partial sums
+    double* red;                    // [q][Bp] reduced sums
+    // observables
+    const int32_t* group;           // [n]
+    int ngroups, site_mode, every;
+    int64_t max_steps;
+    double* pops;                   // [rec][g][Bp] or [rec][i][Bp] (site mode)
+    const double2* bra;             // [n] or null
+    double2* corr;                  // [max_steps+1][Bp]
+    int64_t* step;                  // device step counter
+};
+
+enum StatusBits { kStatusScaling = 1, kStatusNonfinite = 2 };
+
+// launch wrappers (dq_kernels.hip)
+hipError_t launch_init_disorder(const Dev& d, hipStream_t s);
+hipError_t launch_prologue(const Dev& d, int renorm, int noise, int record, int copy_v0, hipStream_t s);
+hipError_t launch_spmm(const Dev& d, int j, int last, hipStream_t s);
+hipError_t launch_scalars_h(const Dev& d, int j, hipStream_t s);
+hipError_t launch_update(const Dev& d, int j, hipStream_t s);
+hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s);
+hipError_t launch_pade(const Dev& d, hipStream_t s);
+hipError_t launch_combine(const Dev& d, hipStream_t s);
+hipError_t launch_finish_norm(const Dev& d, int advance_step, hipStream_t s);
+hipError_t launch_reduce_obs(const Dev& d, hipStream_t s);
+hipError_t launch_set_unit_norm(const Dev& d, hipStream_t s);
+hipError_t launch_advance_step(const Dev& d, hipStream_t s);
+hipError_t launch_pade_tbar(int m, int count, const double* alpha, const double* beta,
+                            double theta, double2* E, int* s_out, hipStream_t s);
+hipError_t launch_fft(int n, int batch, double2* a, double2* b, int sign, hipStream_t s);
+hipError_t launch_spectrum_prep(const Dev& d, int64_t T, double dt, double sigma, int nfft,
+                                double2* buf, hipStream_t s);
+
+}  // namespace dq

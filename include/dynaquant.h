@@ -1,0 +1,188 @@
+/*
+ * dynaquant.h -- C ABI of the MI355X (gfx950) stochastic exciton-dynamics
+ * engine.  Plain C types only: interleaved (re, im) doubles for complex
+ * vectors, int32 indices, int64 counts, opaque handles.
+ *
+ * The entry points replace the seams of the reference's one-executable
+ * driver (propagate.cu:199-260):
+ *   systemClass::parseInFile / readInHam     -> dq_system_create,
+ *                                               dq_system_set_hamiltonian_csr
+ *   systemClass::prepForKA                   -> dq_system_set_ou_params
+ *   systemClass::resetStaticDisorder         -> dq_system_reset_disorder
+ *   systemClass::setInitState                -> dq_system_set_state
+ *   systemClass::updateHamiltonian           -> dq_update_noise
+ *   lanczosClass::doLanczos + padeClass::doPade + propagateSystem
+ *     + renormalizeAndStoreNorm + calculateErrorEstimate
+ *                                            -> dq_propagate_step, dq_run
+ *   popPrintClass::printOneLoopAllPops       -> dq_get_populations
+ *   spectraClass::calcCorrFunc               -> dq_get_corr
+ *   spectraClass::singleSpectraCalc          -> dq_spectrum
+ * Every call returns a dq_status; the message of the last failure on the
+ * calling thread is returned by dq_last_error().  Nothing prints, nothing
+ * exits.
+ */
+#ifndef DYNAQUANT_H
+#define DYNAQUANT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dq_status {
+    DQ_OK = 0,
+    DQ_ERR_INVALID_ARG = 1,
+    DQ_ERR_OOM = 2,
+    DQ_ERR_HIP = 3,
+    DQ_ERR_BREAKDOWN = 4,        /* reserved: breakdowns are handled, not fatal */
+    DQ_ERR_NONFINITE = 5,
+    DQ_ERR_SCALING_OVERFLOW = 6, /* Pade scaling exponent s >= 50 (padeClass.cu:618-623) */
+    DQ_ERR_NO_DEVICE = 7,
+    DQ_ERR_STATE = 8             /* call out of order (e.g. run before set_state) */
+} dq_status;
+
+typedef enum dq_precision {
+    DQ_PREC_FP64 = 0,   /* complex128 vectors, FP64 everything */
+    DQ_PREC_MIXED = 1   /* complex64 Krylov vectors and SpMM; FP64 psi, alpha/beta,
+                           reorthogonalisation sums, Pade */
+} dq_precision;
+
+typedef struct dq_system dq_system;
+
+/* infile.in keys of the reference in brackets (systemClass.cu:231-630) */
+typedef struct dq_config {
+    int32_t n_sites;          /* N [hamfile size]; ragged batches: the largest N */
+    int32_t n_traj;           /* B trajectories in one batch; 1 = plain path */
+    int32_t krylov_m;         /* [m] Krylov depth, 2..32 */
+    int32_t krylov_substeps;  /* K Krylov cycles per dt; 1 = the reference */
+    int32_t ou_substeps;      /* [ka_substeps] Euler-Maruyama substeps per dt */
+    int32_t precision;        /* dq_precision */
+    int32_t force_batched;    /* 1: run B=1 through the batched kernels */
+    int32_t reserved;
+    double dt_fs;             /* [dt] */
+    double hbar_cm_fs;        /* <= 0: the reference's 5308.84 (cppUtility.cpp:10-35) */
+    double coupling_scale;    /* g: H = diag + g * couplings */
+    double sigma_static;      /* [sigmastatdis] static disorder; 0 = [staticdis]=0 */
+    double breakdown_tol;     /* <= 0: 1e-10 (FP64) / 1e-5 (MIXED) */
+    uint64_t seed;            /* Philox key word 0 */
+} dq_config;
+
+/* ---- lifetime -------------------------------------------------------- */
+dq_status dq_system_create(const dq_config *cfg, dq_system **out);
+dq_status dq_system_destroy(dq_system *sys);
+const char *dq_last_error(void);
+dq_status dq_sync(dq_system *sys);
+
+/* ---- system set-up ---------------------------------------------------- */
+/* Off-diagonal couplings (cm^-1, real, row-sorted CSR without the diagonal)
+ * and site energies e_site[n] (replaces the COO file read and cusparse
+ * coo2csr of sparseMatrixClasses.cu:74-267, 489-540; the diagonal index map
+ * of systemClass.cu:808-870 is not needed: the diagonal is held apart). */
+dq_status dq_system_set_hamiltonian_csr(dq_system *sys, int32_t n, const int32_t *rowptr,
+                                        const int32_t *col, const double *val,
+                                        const double *e_site);
+/* Per-site OU relaxation time (fs) and standard deviation (cm^-1)
+ * (ka_paramsfile lines "tau sigma", systemClass.cu:980-1071). */
+dq_status dq_system_set_ou_params(dq_system *sys, const double *tau_fs, const double *sigma);
+/* Philox key word 1 of each trajectory (NULL: 0..B-1) and its active site
+ * count (NULL: all N).  A trajectory with n_active < N propagates under the
+ * leading n_active x n_active block of H.  Clears observables. */
+dq_status dq_system_set_trajectories(dq_system *sys, const uint64_t *traj_ids,
+                                     const int32_t *n_active);
+/* Draw static disorder and the stationary OU start for every trajectory,
+ * reset the step counter and the error sums (systemClass.cu:759-804,
+ * 1028-1036, 1552-1608). */
+dq_status dq_system_reset_disorder(dq_system *sys);
+/* psi[b][i] interleaved complex, b < B, i < N (systemClass.cu:1325-1508) */
+dq_status dq_system_set_state(dq_system *sys, const double *psi);
+dq_status dq_system_get_state(dq_system *sys, double *psi);
+/* Observables: site i adds |psi_i|^2 to group[i] (< 0: none), recorded
+ * every `every` steps from step 0 up to max_steps; when ngroups == N and
+ * group[i] == i the per-site populations are stored directly
+ * (popPrintClass.cu:442-472).  bra[i] (interleaved complex, may be NULL)
+ * enables C(n) = <bra|psi(n)> every step (spectraClass.cu:509-571). */
+dq_status dq_system_set_observables(dq_system *sys, const int32_t *group, int32_t ngroups,
+                                    int32_t every, int64_t max_steps, const double *bra);
+
+/* ---- Hamiltonian generator -------------------------------------------- */
+/* Cylindrical aggregate of point dipoles: wall w has rings[w] rings of
+ * per_ring sites at radius_nm[w]; ring r sits at z = r * ring_dz_nm and odd
+ * rings are rotated by half a site; dipoles of mu_debye are tilted tilt_deg
+ * from the ring tangent towards the axis; J_ij = kappa (mu_i.mu_j -
+ * 3 (mu_i.r)(mu_j.r)) / r^3 for r < cutoff_nm.  order 0 numbers sites wall by
+ * wall, order 1 ring by ring across walls (then every shorter tube with the
+ * same walls is a leading block).  The reference instead reads a
+ * precomputed Hamiltonian file (sparseMatrixClasses.cu:74-267). */
+typedef struct dq_tube_spec {
+    int32_t n_walls;
+    int32_t per_ring;
+    int32_t order;
+    int32_t rings[4];
+    double radius_nm[4];
+    double ring_dz_nm;
+    double mu_debye;
+    double tilt_deg;
+    double cutoff_nm;
+    double kappa;        /* cm^-1 nm^3 / D^2 */
+} dq_tube_spec;
+/* pos[3n], dip[3n] (may be NULL), wall[n] (may be NULL); *n_out = n */
+dq_status dq_tube_sites(const dq_tube_spec *spec, double *pos, double *dip, int32_t *wall,
+                        int32_t *n_out);
+/* Exactly symmetric CSR of the couplings, sorted columns, no diagonal.
+ * With col == NULL only rowptr[n+1] and *nnz_out are filled. */
+dq_status dq_tube_couplings(const dq_tube_spec *spec, int32_t *rowptr, int32_t *col, double *val,
+                            int64_t *nnz_out);
+
+/* ---- propagation ------------------------------------------------------ */
+/* Advance the OU noise of the current step, write the diagonal and move
+ * the step counter on without propagating (systemClass.cu:1881-1971,
+ * gpuSysHelper.cu:73-128).  A unit seam: dq_run includes the noise. */
+dq_status dq_update_noise(dq_system *sys);
+/* One dt for all trajectories: noise, K x (Lanczos, Pade, combine,
+ * renormalise, Expokit error), observables. */
+dq_status dq_propagate_step(dq_system *sys);
+/* nsteps steps, then renormalise and record the final state; returns when
+ * the device is done.  use_graph != 0 replays one captured step as a
+ * hipGraph (one graph launch per dt). */
+dq_status dq_run(dq_system *sys, int64_t nsteps, int32_t use_graph);
+dq_status dq_get_step(dq_system *sys, int64_t *step);
+
+/* ---- results ---------------------------------------------------------- */
+/* pops[b][k][g] for records k = 0..step/every (*nrec set to that count). */
+dq_status dq_get_populations(dq_system *sys, double *pops, int64_t *nrec);
+/* corr[b][n] interleaved complex, n = 0..step */
+dq_status dq_get_corr(dq_system *sys, double *corr);
+/* sum over cycles of the clamped Expokit estimate, per trajectory */
+dq_status dq_get_error_sum(dq_system *sys, double *err);
+
+/* ---- unit seams (for tests and tools) --------------------------------- */
+/* Current diagonal d[b][i] = static energy + OU offset of every trajectory
+ * (the values updateHamKuboAnderson writes into the CSR diagonal,
+ * gpuSysHelper.cu:121-123); FP32 in MIXED precision, returned as double. */
+dq_status dq_get_diagonal(dq_system *sys, double *diag);
+/* y = H_b x for trajectory b with its current diagonal (lanczosClass.cu:618-661) */
+dq_status dq_spmv(dq_system *sys, int32_t traj, const double *x, double *y);
+/* exp(-i theta Tbar) for `count` augmented tridiagonals (alpha[c][m],
+ * beta[c][m]), complex column-major E[c][(m+2)^2]; s[c] = scaling exponent.
+ * Runs the engine's Pade kernel (padeClass.cu:204-487). */
+dq_status dq_pade_tbar(int32_t m, int32_t count, const double *alpha, const double *beta,
+                       double theta, double *E, int32_t *s);
+/* Absorption spectrum from the batch-averaged C(n) (spectraClass.cu:575-750):
+ * window exp(-0.5((n+1)dt/sigma)^2) for n < T (sigma <= 0: none),
+ * zero-pad to nfft (power of two), inverse-direction FFT on the GPU,
+ * scale mu2/T.  omega[nfft] in cm^-1, spec[nfft] interleaved complex. */
+dq_status dq_spectrum(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
+                      double *omega, double *spec);
+/* In-place FFTs of `batch` interleaved complex rows of length n (power of
+ * two, <= 65536) on the GPU (replaces cufftExecZ2Z, spectraClass.cu:617). */
+dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
+
+/* ---- timing ----------------------------------------------------------- */
+/* Milliseconds of the last dq_run, measured with HIP events on the
+ * system's stream. */
+dq_status dq_last_run_ms(dq_system *sys, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,489 @@
+/*
+ * krylov.c -- oracle restatement of the propagation step.  TEST INFRASTRUCTURE.
+ *
+ * One time step (propagate.cu:199-260):
+ *   noise update of the diagonal       systemClass.cu:1940-1971, gpuSysHelper.cu:73-128
+ *   Lanczos from psi                   lanczosClass.cu:230-353, 395-531
+ *   augmented Tbar                     gpuLanczosHelper.cu:14-60
+ *   Pade(6,6) scaling and squaring     padeClass.cu:204-487, 584-745
+ *   psi = Q_m exp(A)[0:m,0]            propStep.cu:31-44
+ *   renormalise                        systemClass.cu:1769-1829
+ *   Expokit error estimate             padeClass.cu:749-782, systemClass.cu:2055-2085
+ * with the deviations the engine makes on purpose: Hermitian (conjugated)
+ * alpha instead of cublasZdotu (lanczosClass.cu:421), the true infinity norm
+ * sum |A_ij| instead of sum |Re A_ij| (gpuPadeHelper.cu:25), optional full
+ * reorthogonalisation, a breakdown guard, and K Krylov substeps per dt.
+ */
+#include "dq_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+double orc_hbar_ref(void)
+{
+    /* cppUtility.cpp:19-22 evaluated in the same order */
+    double pi = atan(1) * 4;
+    double lightspeed = 299792548.;
+    return (1 / 100.) * (1e15) * 1. / (lightspeed * 2 * pi);
+}
+
+void orc_pade_coeffs(int p, double *c)
+{
+    /* padeClass.cu:161-168 */
+    c[0] = 1;
+    double fp = (double)p;
+    for (int k = 1; k <= p; ++k) {
+        double fk = (double)k;
+        c[k] = c[k - 1] * (fp + 1. - fk) / ((2 * fp + 1. - fk) * fk);
+    }
+}
+
+void orc_spmv(int n, const int32_t *rowptr, const int32_t *col,
+              const double *val, double g, const double *diag,
+              const double *x, double *y)
+{
+    for (int i = 0; i < n; ++i) {
+        double ar = 0.0, ai = 0.0;
+        for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+            int j = col[p];
+            ar += val[p] * x[2 * j];
+            ai += val[p] * x[2 * j + 1];
+        }
+        y[2 * i] = diag[i] * x[2 * i] + g * ar;
+        y[2 * i + 1] = diag[i] * x[2 * i + 1] + g * ai;
+    }
+}
+
+/* <a|b> = sum conj(a_i) b_i */
+static void cdot(int n, const double *a, const double *b, double *re, double *im)
+{
+    double sr = 0.0, si = 0.0;
+    for (int i = 0; i < n; ++i) {
+        sr += a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1];
+        si += a[2 * i] * b[2 * i + 1] - a[2 * i + 1] * b[2 * i];
+    }
+    *re = sr;
+    *im = si;
+}
+
+static double cnorm(int n, const double *a)
+{
+    double s = 0.0;
+    for (int i = 0; i < 2 * n; ++i) s += a[i] * a[i];
+    return sqrt(s);
+}
+
+int orc_lanczos(int n, const int32_t *rowptr, const int32_t *col,
+                const double *val, double g, const double *diag,
+                const double *psi, const orc_lanczos_opts *o,
+                double *Q, double *alpha, double *beta, double *nrm_hq)
+{
+    const int m = o->m;
+    const size_t cs = 2 * (size_t)n;
+    double *w = (double *)malloc(cs * sizeof(double));
+    /* projection coefficients h^(j)_k (complex) and Gram entries G[l][j] */
+    double *hh = (double *)calloc((size_t)2 * (m + 2) * (m + 2), sizeof(double));
+    double *G = (double *)calloc((size_t)2 * (m + 2) * (m + 2), sizeof(double));
+#define HH(j, k) (hh + 2 * ((size_t)(j) * (m + 2) + (k)))
+#define GG(l, j) (G + 2 * ((size_t)(l) * (m + 2) + (j)))
+    memset(Q, 0, cs * (m + 2) * sizeof(double));
+    memcpy(Q + cs, psi, cs * sizeof(double));
+    for (int j = 0; j <= m; ++j) { alpha[j] = 0.0; beta[j] = 0.0; }
+    GG(1, 1)[0] = 1.0;
+    int broke = 0;
+    for (int j = 1; j <= m; ++j) {
+        double *qj = Q + cs * j;
+        orc_spmv(n, rowptr, col, val, g, diag, qj, w);
+        if (o->reorth == ORC_REORTH_NONE) {
+            double ar, ai;
+            cdot(n, qj, w, &ar, &ai);
+            alpha[j] = ar;
+            const double *qm = Q + cs * (j - 1);
+            for (size_t i = 0; i < cs; ++i) w[i] = w[i] - ar * qj[i] - beta[j - 1] * qm[i];
+        } else {
+            double *h = HH(j, 0);
+            cdot(n, qj, w, &h[2 * j], &h[2 * j + 1]);
+            if (o->reorth == ORC_REORTH_CGS) {
+                for (int k = 1; k < j; ++k)
+                    cdot(n, Q + cs * k, w, &h[2 * k], &h[2 * k + 1]);
+            } else {
+                /* q_k^H H q_j = (H q_k)^H q_j with H q_k = sum_l h^(k)_l q_l
+                 * + beta_k q_{k+1}: projections from the Gram row of q_j. */
+                for (int k = 1; k < j; ++k) {
+                    double sr = beta[k] * GG(k + 1, j)[0];
+                    double si = beta[k] * GG(k + 1, j)[1];
+                    for (int l = 1; l <= k; ++l) {
+                        const double *a = HH(k, l), *b = GG(l, j);
+                        sr += a[0] * b[0] + a[1] * b[1];
+                        si += a[0] * b[1] - a[1] * b[0];
+                    }
+                    h[2 * k] = sr;
+                    h[2 * k + 1] = si;
+                }
+            }
+            alpha[j] = h[2 * j];
+            for (int k = 1; k <= j; ++k) {
+                const double *qk = Q + cs * k;
+                double hr = h[2 * k], hi = h[2 * k + 1];
+                for (int i = 0; i < n; ++i) {
+                    double xr = qk[2 * i], xi = qk[2 * i + 1];
+                    w[2 * i] -= hr * xr - hi * xi;
+                    w[2 * i + 1] -= hr * xi + hi * xr;
+                }
+            }
+        }
+        double b = cnorm(n, w);
+        double *qn = Q + cs * (j + 1);
+        if (!(b > o->breakdown_tol * (fabs(alpha[j]) + beta[j - 1]))) {
+            broke = 1;
+            beta[j] = 0.0;
+            break;                          /* q_{j+1..} stay zero */
+        }
+        beta[j] = b;
+        double s = 1.0 / b;
+        for (size_t i = 0; i < cs; ++i) qn[i] = w[i] * s;
+        if (o->reorth == ORC_REORTH_GRAM) {
+            for (int l = 1; l <= j; ++l)
+                cdot(n, Q + cs * l, qn, &GG(l, j + 1)[0], &GG(l, j + 1)[1]);
+            GG(j + 1, j + 1)[0] = 1.0;
+            GG(j + 1, j + 1)[1] = 0.0;
+        }
+    }
+    if (!broke) {
+        orc_spmv(n, rowptr, col, val, g, diag, Q + cs * (m + 1), w);
+        *nrm_hq = cnorm(n, w);
+    } else {
+        *nrm_hq = 0.0;
+    }
+#undef HH
+#undef GG
+    free(w);
+    free(hh);
+    free(G);
+    return broke;
+}
+
+void orc_build_tbar(int m, const double *alpha, const double *beta, double *tb)
+{
+    const int n = m + 2;
+    memset(tb, 0, sizeof(double) * n * n);
+    for (int i = 0; i < m; ++i) {
+        tb[i + i * n] = alpha[i + 1];
+        if (i + 1 < m) {
+            tb[i + (i + 1) * n] = beta[i + 1];
+            tb[(i + 1) + i * n] = beta[i + 1];
+        }
+    }
+    tb[0 + m * n] = 1.0;
+    tb[m + (m + 1) * n] = 1.0;
+}
+
+/* C = A B, complex column-major n x n */
+static void cmm(int n, const double *A, const double *B, double *C)
+{
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double *a = A + 2 * (i + k * n), *b = B + 2 * (k + j * n);
+                sr += a[0] * b[0] - a[1] * b[1];
+                si += a[0] * b[1] + a[1] * b[0];
+            }
+            C[2 * (i + j * n)] = sr;
+            C[2 * (i + j * n) + 1] = si;
+        }
+}
+
+/* Solve D X = B (complex, n x n, nrhs = n) in place in B by Gaussian
+ * elimination with partial pivoting on |re|+|im| (culaDeviceZgesv role,
+ * padeClass.cu:714-745).  D is destroyed. */
+static int csolve(int n, double *D, double *B)
+{
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = -1.0;
+        for (int i = k; i < n; ++i) {
+            double v = fabs(D[2 * (i + k * n)]) + fabs(D[2 * (i + k * n) + 1]);
+            if (v > best) { best = v; p = i; }
+        }
+        if (!(best > 0.0)) return -1;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) {
+                double *a = D + 2 * (k + j * n), *b = D + 2 * (p + j * n);
+                double t0 = a[0], t1 = a[1]; a[0] = b[0]; a[1] = b[1]; b[0] = t0; b[1] = t1;
+                a = B + 2 * (k + j * n); b = B + 2 * (p + j * n);
+                t0 = a[0]; t1 = a[1]; a[0] = b[0]; a[1] = b[1]; b[0] = t0; b[1] = t1;
+            }
+        }
+        double pr = D[2 * (k + k * n)], pi = D[2 * (k + k * n) + 1];
+        double den = pr * pr + pi * pi;
+        double ir = pr / den, ii = -pi / den;       /* 1/pivot */
+        for (int i = k + 1; i < n; ++i) {
+            double *a = D + 2 * (i + k * n);
+            double lr = a[0] * ir - a[1] * ii, li = a[0] * ii + a[1] * ir;
+            a[0] = lr; a[1] = li;
+            for (int j = k + 1; j < n; ++j) {
+                double *d = D + 2 * (i + j * n);
+                const double *u = D + 2 * (k + j * n);
+                d[0] -= lr * u[0] - li * u[1];
+                d[1] -= lr * u[1] + li * u[0];
+            }
+            for (int j = 0; j < n; ++j) {
+                double *d = B + 2 * (i + j * n);
+                const double *u = B + 2 * (k + j * n);
+                d[0] -= lr * u[0] - li * u[1];
+                d[1] -= lr * u[1] + li * u[0];
+            }
+        }
+    }
+    for (int j = 0; j < n; ++j) {
+        for (int i = n - 1; i >= 0; --i) {
+            double sr = B[2 * (i + j * n)], si = B[2 * (i + j * n) + 1];
+            for (int k = i + 1; k < n; ++k) {
+                const double *u = D + 2 * (i + k * n), *x = B + 2 * (k + j * n);
+                sr -= u[0] * x[0] - u[1] * x[1];
+                si -= u[0] * x[1] + u[1] * x[0];
+            }
+            double pr = D[2 * (i + i * n)], pi = D[2 * (i + i * n) + 1];
+            double den = pr * pr + pi * pi;
+            B[2 * (i + j * n)] = (sr * pr + si * pi) / den;
+            B[2 * (i + j * n) + 1] = (si * pr - sr * pi) / den;
+        }
+    }
+    return 0;
+}
+
+int orc_pade_expm(int n, const double *A_in, double *E, int *s_out)
+{
+    const size_t sz = (size_t)2 * n * n;
+    double c[7];
+    orc_pade_coeffs(6, c);
+    double *A = (double *)malloc(sz * sizeof(double) * 8);
+    double *A2 = A + sz, *A4 = A2 + sz, *A6 = A4 + sz;
+    double *T1 = A6 + sz, *Num = T1 + sz, *Den = Num + sz, *Tmp = Den + sz;
+    memcpy(A, A_in, sz * sizeof(double));
+    /* true infinity norm (doScalingStep, padeClass.cu:584-649, with
+     * |A_ij| instead of |Re A_ij|) */
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double rs = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double re = A[2 * (i + j * n)], im = A[2 * (i + j * n) + 1];
+            rs += sqrt(re * re + im * im);
+        }
+        if (rs > nrm) nrm = rs;
+    }
+    int s = 0;
+    while (nrm > 0.5) { nrm = 0.5 * nrm; s = s + 1; }
+    *s_out = s;
+    if (s >= 50) { free(A); return -1; }
+    double sc = ldexp(1.0, -s);
+    for (size_t i = 0; i < sz; ++i) A[i] *= sc;
+    cmm(n, A, A, A2);
+    cmm(n, A2, A2, A4);
+    cmm(n, A2, A4, A6);
+    /* U = A (c1 I + c3 A^2 + c5 A^4) */
+    for (size_t i = 0; i < sz; ++i) T1[i] = c[3] * A2[i] + c[5] * A4[i];
+    for (int i = 0; i < n; ++i) T1[2 * (i + i * n)] += c[1];
+    cmm(n, A, T1, Num);
+    /* V = c0 I + c2 A^2 + c4 A^4 + c6 A^6 ; Den = V - U */
+    for (size_t i = 0; i < sz; ++i) Den[i] = c[2] * A2[i] + c[4] * A4[i] + c[6] * A6[i] - Num[i];
+    for (int i = 0; i < n; ++i) Den[2 * (i + i * n)] += c[0];
+    memcpy(Tmp, Num, sz * sizeof(double));
+    if (csolve(n, Den, Tmp) != 0) { free(A); return -2; }
+    /* E = I + 2 X, then square s times (padeClass.cu:451-476, 656-706) */
+    for (size_t i = 0; i < sz; ++i) E[i] = 2.0 * Tmp[i];
+    for (int i = 0; i < n; ++i) E[2 * (i + i * n)] += 1.0;
+    for (int k = 0; k < s; ++k) {
+        cmm(n, E, E, Tmp);
+        memcpy(E, Tmp, sz * sizeof(double));
+    }
+    free(A);
+    return 0;
+}
+
+double orc_expokit_err(int m, double beta_m, double nrm_hq, const double *E)
+{
+    const int n = m + 2;
+    double hm = fabs(beta_m);
+    const double *e1 = E + 2 * ((m - 1) + m * n);
+    const double *e2 = E + 2 * ((m - 1) + (m + 1) * n);
+    double err1 = hm * sqrt(e1[0] * e1[0] + e1[1] * e1[1]);
+    double err2 = hm * nrm_hq * sqrt(e2[0] * e2[0] + e2[1] * e2[1]);
+    if (err1 > err2) return err2 / (1 - err2 / err1);
+    return err1;
+}
+
+/* ------------------------------------------------------------------ */
+void orc_init_disorder(const orc_params *p, const orc_site_params *sp,
+                       uint64_t traj, double *e0, double *de)
+{
+    for (int i = 0; i < p->n; ++i) {
+        double base = sp->e_site ? sp->e_site[i] : 0.0;
+        if (p->sigma_static > 0.0) {
+            double z = orc_normal(p->seed, traj, (uint64_t)i, 0, 1, 0);
+            e0[i] = p->sigma_static * z + base;
+        } else {
+            e0[i] = base;
+        }
+        if (sp->e_shift) e0[i] = e0[i] + sp->e_shift[i];
+        double sig = sp->ou_sigma ? sp->ou_sigma[i] : 0.0;
+        de[i] = sig * orc_normal(p->seed, traj, (uint64_t)i, 0, 2, 0);
+    }
+}
+
+void orc_ou_step(const orc_params *p, const orc_site_params *sp,
+                 uint64_t traj, int64_t step, double *de)
+{
+    const int S = p->ou_substeps;
+    double subdt = p->dt_fs / (double)S;
+    double sqrt_subdt = sqrt(subdt);
+    for (int i = 0; i < p->n; ++i) {
+        if (!sp->ou_sigma || !sp->ou_tau || sp->ou_sigma[i] == 0.0) continue;
+        double invTau = 1. / sp->ou_tau[i];
+        double coeff = sp->ou_sigma[i] * sqrt(2. / sp->ou_tau[i]);
+        double cur = de[i];
+        for (int s = 0; s < S; ++s) {
+            double xi = orc_normal(p->seed, traj, (uint64_t)i, (uint64_t)step, 3, (uint64_t)s);
+            cur = cur - cur * invTau * subdt + coeff * xi * sqrt_subdt;
+        }
+        de[i] = cur;
+    }
+}
+
+/* one Krylov cycle psi <- Q_m exp(-i tau Tbar/hbar)[0:m,0]; returns
+ * clamped error estimate, or a negative value on failure */
+static double krylov_cycle(int n, const int32_t *rowptr, const int32_t *col,
+                           const double *val, double g, const double *diag,
+                           const orc_lanczos_opts *lo, double tau, double hbar,
+                           double *psi, double *Q, double *work)
+{
+    const int m = lo->m, nt = m + 2;
+    double *alpha = work, *beta = alpha + (m + 2);
+    double *tb = beta + (m + 2);
+    double *A = tb + nt * nt;
+    double *E = A + 2 * nt * nt;
+    double nrm_hq;
+    orc_lanczos(n, rowptr, col, val, g, diag, psi, lo, Q, alpha, beta, &nrm_hq);
+    orc_build_tbar(m, alpha, beta, tb);
+    double cim = (-1. / hbar) * tau;         /* propagate.cu:91-92 */
+    for (int i = 0; i < nt * nt; ++i) { A[2 * i] = 0.0; A[2 * i + 1] = cim * tb[i]; }
+    int s;
+    if (orc_pade_expm(nt, A, E, &s) != 0) return -1.0;
+    const size_t cs = 2 * (size_t)n;
+    for (int i = 0; i < n; ++i) {
+        double sr = 0.0, si = 0.0;
+        for (int k = 1; k <= m; ++k) {
+            const double *q = Q + cs * k + 2 * i;
+            double yr = E[2 * (k - 1)], yi = E[2 * (k - 1) + 1];
+            sr += yr * q[0] - yi * q[1];
+            si += yr * q[1] + yi * q[0];
+        }
+        psi[2 * i] = sr;
+        psi[2 * i + 1] = si;
+    }
+    double err = orc_expokit_err(m, beta[m], nrm_hq, E);
+    if (err < 1.110223e-16) err = 1.110223e-16;   /* includeFiles.cu:10 */
+    return err;
+}
+
+static void record(int n, const double *psi, const int32_t *group, int ngroups, double *out)
+{
+    for (int g = 0; g < ngroups; ++g) out[g] = 0.0;
+    for (int i = 0; i < n; ++i) {
+        int g = group ? group[i] : i;
+        if (g < 0 || g >= ngroups) continue;
+        out[g] += psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+    }
+}
+
+int orc_run(const orc_params *p, const orc_site_params *sp,
+            const int32_t *rowptr, const int32_t *col, const double *val,
+            const double *psi0, uint64_t traj, int64_t nsteps,
+            int obs_every, const int32_t *group, int ngroups,
+            const double *bra, double *pops, double *corr,
+            double *psi_out, double *err_sum)
+{
+    const int n = p->n, m = p->m, nt = m + 2;
+    const size_t cs = 2 * (size_t)n;
+    double *psi = (double *)malloc(cs * sizeof(double));
+    double *Q = (double *)malloc(cs * (m + 2) * sizeof(double));
+    double *e0 = (double *)malloc(sizeof(double) * n * 3);
+    double *de = e0 + n, *diag = de + n;
+    double *work = (double *)malloc(sizeof(double) * (2 * (m + 2) + nt * nt * 5));
+    orc_lanczos_opts lo = {m, p->reorth, p->breakdown_tol};
+    memcpy(psi, psi0, cs * sizeof(double));
+    orc_init_disorder(p, sp, traj, e0, de);
+    double tau = p->dt_fs / (double)p->krylov_substeps;
+    double esum = 0.0;
+    int rc = 0;
+    if (pops && obs_every > 0) record(n, psi, group, ngroups, pops);
+    if (corr && bra) cdot(n, bra, psi, &corr[0], &corr[1]);
+    for (int64_t step = 0; step < nsteps; ++step) {
+        orc_ou_step(p, sp, traj, step, de);
+        for (int i = 0; i < n; ++i) diag[i] = de[i] + e0[i];
+        for (int k = 0; k < p->krylov_substeps; ++k) {
+            double err = krylov_cycle(n, rowptr, col, val, p->g, diag, &lo, tau,
+                                      p->hbar, psi, Q, work);
+            if (err < 0.0) { rc = -1; goto done; }
+            esum += err;
+            double nr = cnorm(n, psi);
+            double inv = 1.0 / nr;
+            for (size_t i = 0; i < cs; ++i) psi[i] *= inv;
+        }
+        if (pops && obs_every > 0 && (step + 1) % obs_every == 0)
+            record(n, psi, group, ngroups, pops + (size_t)ngroups * ((step + 1) / obs_every));
+        if (corr && bra) cdot(n, bra, psi, &corr[2 * (step + 1)], &corr[2 * (step + 1) + 1]);
+    }
+done:
+    if (psi_out) memcpy(psi_out, psi, cs * sizeof(double));
+    if (err_sum) *err_sum = esum;
+    free(psi); free(Q); free(e0); free(work);
+    return rc;
+}
+
+int orc_run_batch(const orc_params *p, const orc_site_params *sp,
+                  const int32_t *rowptr, const int32_t *col, const double *val,
+                  const double *psi0, const uint64_t *traj_ids, int nb,
+                  const int32_t *n_active, int64_t nsteps, int obs_every,
+                  const int32_t *group, int ngroups, const double *bra,
+                  double *pops, double *corr, double *err_sum, int nthreads)
+{
+    const int n = p->n;
+    const size_t nobs = obs_every > 0 ? (size_t)(nsteps / obs_every + 1) : 0;
+    int fail = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(| : fail)
+#endif
+    for (int b = 0; b < nb; ++b) {
+        int nb_act = n_active ? n_active[b] : n;
+        orc_params pb = *p;
+        pb.n = nb_act;
+        /* leading nb_act x nb_act block of the coupling matrix */
+        int32_t *rp = (int32_t *)malloc(sizeof(int32_t) * (nb_act + 1));
+        int64_t cnt = 0;
+        rp[0] = 0;
+        for (int i = 0; i < nb_act; ++i) {
+            for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) if (col[q] < nb_act) ++cnt;
+            rp[i + 1] = (int32_t)cnt;
+        }
+        int32_t *cl = (int32_t *)malloc(sizeof(int32_t) * (cnt + 1));
+        double *vl = (double *)malloc(sizeof(double) * (cnt + 1));
+        cnt = 0;
+        for (int i = 0; i < nb_act; ++i)
+            for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
+                if (col[q] < nb_act) { cl[cnt] = col[q]; vl[cnt] = val[q]; ++cnt; }
+        int r = orc_run(&pb, sp, rp, cl, vl, psi0 + (size_t)b * 2 * n, traj_ids[b], nsteps,
+                        obs_every, group, ngroups, bra,
+                        pops ? pops + (size_t)b * nobs * ngroups : NULL,
+                        corr ? corr + (size_t)b * 2 * (nsteps + 1) : NULL,
+                        NULL, err_sum ? err_sum + b : NULL);
+        if (r != 0) fail = 1;
+        free(rp); free(cl); free(vl);
+    }
+    return fail ? -1 : 0;
+}
