@@ -25,8 +25,11 @@ EXPORTS = [
     "dq_system_set_observables", "dq_tube_sites", "dq_tube_couplings", "dq_update_noise",
     "dq_propagate_step", "dq_run", "dq_get_step", "dq_get_populations", "dq_get_corr",
     "dq_get_error_sum", "dq_get_diagonal", "dq_spmv", "dq_pade_tbar", "dq_spectrum", "dq_fft",
-    "dq_last_run_ms",
+    "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times",
 ]
+
+KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
+                  "combine", "norm"]
 
 
 class DQError(RuntimeError):
@@ -209,6 +212,16 @@ class System:
         ms = C.c_double()
         _check(lib().dq_last_run_ms(self._h, C.byref(ms)))
         return ms.value
+
+    def set_profiling(self, on=True):
+        _check(lib().dq_set_profiling(self._h, int(on)))
+
+    def kernel_times(self):
+        """{class: (total ms, launches)} accumulated in profiling mode"""
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+        _check(lib().dq_get_kernel_times(self._h, _p(ms), _p(cnt)))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
     def get_step(self):
         s = C.c_int64()

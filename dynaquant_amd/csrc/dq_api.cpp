@@ -68,6 +68,13 @@ struct dq_system {
     // host copies
     std::vector<int32_t> rowptr, col;
     std::vector<double> val;
+    // profiling: (class, start, stop) per bracketed launch of the current run
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_class;
+    size_t ev_used = 0;
+    double kms[DQ_K_CLASSES] = {0};
+    int64_t kcount[DQ_K_CLASSES] = {0};
     // observable config
     int ngroups = 0, every = 0, site_mode = 0;
     int64_t max_steps = 0;
@@ -76,6 +83,7 @@ struct dq_system {
     ~dq_system()
     {
         if (graph) hipGraphExecDestroy(graph);
+        for (hipEvent_t e : ev_pool) hipEventDestroy(e);
         for (void* p : allocs) hipFree(p);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -432,24 +440,65 @@ dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_
 // ---------------------------------------------------------------------------
 // step sequence
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
+// brackets one launch with events when profiling
+template <typename F>
+static hipError_t timed(dq_system* sys, int cls, F&& launch)
+{
+    if (!sys->profiling) return launch();
+    while (sys->ev_pool.size() < sys->ev_used + 2) {
+        hipEvent_t ev;
+        hipError_t e = hipEventCreate(&ev);
+        if (e != hipSuccess) return e;
+        sys->ev_pool.push_back(ev);
+    }
+    hipEvent_t a = sys->ev_pool[sys->ev_used], b = sys->ev_pool[sys->ev_used + 1];
+    sys->ev_used += 2;
+    sys->ev_class.push_back(cls);
+    hipError_t e = hipEventRecord(a, sys->stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(b, sys->stream);
+    return e;
+}
+
+static hipError_t collect_timings(dq_system* sys)
+{
+    if (!sys->profiling) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(sys->stream);
+    if (e != hipSuccess) return e;
+    for (size_t k = 0; k < sys->ev_class.size(); ++k) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, sys->ev_pool[2 * k], sys->ev_pool[2 * k + 1]);
+        if (e != hipSuccess) return e;
+        sys->kms[sys->ev_class[k]] += ms;
+        sys->kcount[sys->ev_class[k]] += 1;
+    }
+    sys->ev_class.clear();
+    sys->ev_used = 0;
+    return hipSuccess;
+}
+
+extern "C" {
+
 static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
 {
     const Dev& d = sys->d;
     hipStream_t st = sys->stream;
     hipError_t e;
     const bool obs = (d.every > 0 || d.bra != nullptr);
-    if ((e = dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st)) != hipSuccess) return e;
-    if (first_in_step && obs && (e = dq::launch_reduce_obs(d, st)) != hipSuccess) return e;
+    if ((e = timed(sys, DQ_K_PROLOGUE, [&] { return dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st); })) != hipSuccess) return e;
+    if (first_in_step && obs && (e = timed(sys, DQ_K_NORM, [&] { return dq::launch_reduce_obs(d, st); })) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
-        if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
-        if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
-        if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
-        if ((e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;
+        if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_spmm(d, j, 0, st); })) != hipSuccess) return e;
+        if ((e = timed(sys, DQ_K_SCALARS, [&] { return dq::launch_scalars_h(d, j, st); })) != hipSuccess) return e;
+        if ((e = timed(sys, DQ_K_UPDATE, [&] { return dq::launch_update(d, j, st); })) != hipSuccess) return e;
+        if ((e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
     }
-    if ((e = dq::launch_spmm(d, d.m, 1, st)) != hipSuccess) return e;
-    if ((e = dq::launch_pade(d, st)) != hipSuccess) return e;
-    if ((e = dq::launch_combine(d, st)) != hipSuccess) return e;
-    return dq::launch_finish_norm(d, last_in_step ? 1 : 0, st);
+    if ((e = timed(sys, DQ_K_SPMM_LAST, [&] { return dq::launch_spmm(d, d.m, 1, st); })) != hipSuccess) return e;
+    if ((e = timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); })) != hipSuccess) return e;
+    if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_combine(d, st); })) != hipSuccess) return e;
+    return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, last_in_step ? 1 : 0, st); });
 }
 
 static hipError_t enqueue_step(dq_system* sys)
@@ -511,19 +560,39 @@ dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
         hipGraphDestroy(g);
         if (e != hipSuccess) { sys->graph = nullptr; return fail(DQ_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
     }
+    if (sys->profiling) use_graph = 0;
     HIP_TRY(hipEventRecord(sys->ev0, st));
     for (int64_t k = 0; k < nsteps; ++k) {
         if (use_graph) HIP_TRY(hipGraphLaunch(sys->graph, st));
         else HIP_TRY(enqueue_step(sys));
+        if (sys->profiling && sys->ev_used > 4096) HIP_TRY(collect_timings(sys));
     }
     HIP_TRY(hipEventRecord(sys->ev1, st));
     HIP_TRY(enqueue_finalize(sys));
     HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(collect_timings(sys));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
     sys->last_ms = ms;
     sys->step += nsteps;
     return dq_sync(sys);
+}
+
+dq_status dq_set_profiling(dq_system* sys, int32_t on)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    sys->profiling = on != 0;
+    for (int c = 0; c < DQ_K_CLASSES; ++c) { sys->kms[c] = 0.0; sys->kcount[c] = 0; }
+    sys->ev_class.clear();
+    sys->ev_used = 0;
+    return DQ_OK;
+}
+
+dq_status dq_get_kernel_times(dq_system* sys, double* ms, int64_t* launches)
+{
+    if (!sys || !ms || !launches) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    for (int c = 0; c < DQ_K_CLASSES; ++c) { ms[c] = sys->kms[c]; launches[c] = sys->kcount[c]; }
+    return DQ_OK;
 }
 
 dq_status dq_get_step(dq_system* sys, int64_t* step)

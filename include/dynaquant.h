@@ -181,6 +181,16 @@ dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
 /* Milliseconds of the last dq_run, measured with HIP events on the
  * system's stream. */
 dq_status dq_last_run_ms(dq_system *sys, double *ms);
+/* Kernel classes timed by the profiling mode. */
+enum {
+    DQ_K_PROLOGUE = 0, DQ_K_SPMM = 1, DQ_K_SCALARS = 2, DQ_K_UPDATE = 3, DQ_K_REDUCE = 4,
+    DQ_K_SPMM_LAST = 5, DQ_K_PADE = 6, DQ_K_COMBINE = 7, DQ_K_NORM = 8, DQ_K_CLASSES = 9
+};
+/* on != 0: later dq_run calls launch eagerly and bracket every kernel with
+ * HIP events on the system's stream; times accumulate per class until the
+ * mode is switched on again.  ms[DQ_K_CLASSES], launches[DQ_K_CLASSES]. */
+dq_status dq_set_profiling(dq_system *sys, int32_t on);
+dq_status dq_get_kernel_times(dq_system *sys, double *ms, int64_t *launches);
 
 #ifdef __cplusplus
 }
