@@ -324,3 +324,33 @@ def spectrum(corr, dt, sigma, mu2, nfft):
     if lib().orc_spectrum(T, c2i(corr), dt, sigma, mu2, nfft, om, sp) != 0:
         raise ValueError("bad spectrum args")
     return om, i2c(sp)
+
+
+def adjoint(p, sites, rowptr, col, val, psi0, traj, nsteps, group=None, ngroups=0, w_pop=None,
+            bra=None, w_spec=0.0, corr_target=None, ckpt=50):
+    """(loss, dL/d e_site[n], dL/dg) by the exact reverse sweep"""
+    n = p.n
+    grp = None if group is None else _c(np.asarray(group, np.int32))
+    wp = None if w_pop is None else _c(np.asarray(w_pop, np.float64))
+    braa = None if bra is None else c2i(bra)
+    ct = None if corr_target is None else c2i(corr_target)
+    ge = np.zeros(n)
+    gg = C.c_double()
+    loss = lib().orc_adjoint(C.byref(p), C.byref(sites.sp), _c(rowptr), _c(col), _c(val), c2i(psi0),
+                             traj, nsteps, _ptr(grp), ngroups, _ptr(wp), _ptr(braa), w_spec, _ptr(ct),
+                             ckpt, ge, C.byref(gg))
+    return loss, ge, gg.value
+
+
+def loss_fd(p, sites_kw, rowptr, col, val, psi0, traj, nsteps, group, ngroups, w_pop,
+            bra=None, w_spec=0.0, corr_target=None, e_shift=None):
+    """the same loss evaluated by orc_run (for finite differences)"""
+    n = p.n
+    s = Sites(n, e_shift=e_shift, **sites_kw)
+    r = run(p, s, rowptr, col, val, psi0, traj, nsteps, obs_every=nsteps, group=group,
+            ngroups=ngroups, bra=bra)
+    L = float(np.dot(w_pop, r["pops"][-1])) if w_pop is not None else 0.0
+    if bra is not None and w_spec != 0.0:
+        ct = 0.0 if corr_target is None else np.asarray(corr_target)
+        L += w_spec * float(np.sum(np.abs(r["corr"] - ct) ** 2))
+    return L
