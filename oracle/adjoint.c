@@ -361,6 +361,60 @@ static int cycle_forward(cycle_t *c, const double *diag, const double *psi, doub
     return 0;
 }
 
+/* Reverse of the Lanczos part by the three-term recurrence
+ * w' = H q_j - alpha_j q_j - beta_{j-1} q_{j-1}, using the stored
+ * (reorthogonalised) basis and H q_j = beta_j q_{j+1} + alpha_j q_j +
+ * beta_{j-1} q_{j-1}.  The full-projection map and the three-term map are
+ * the same function of psi in exact arithmetic, so their derivatives agree
+ * up to the basis' loss of orthogonality; this costs O(m) vector operations
+ * per cycle instead of O(m^2).  Qb[k] holds the adjoints of q_k from the
+ * combine; ab/bb the adjoints of alpha/beta from the Pade. */
+static void lanczos_reverse_3term(cycle_t *c, const double *diag, double *Qb, const double *ab,
+                                  double *bb, double *dbar, double *gbar)
+{
+    const int n = c->n;
+    const size_t cs = 2 * (size_t)n;
+    double *wb = (double *)malloc(cs * sizeof(double));
+    double *Hw = (double *)malloc(cs * sizeof(double));
+    for (int j = c->jb; j >= 1; --j) {
+        const double *qn = c->Q + cs * (j + 1);
+        const double *q = c->Q + cs * j;
+        const double *qp = c->Q + cs * (j - 1);
+        const double bj = c->beta[j], bp = c->beta[j - 1], aj = c->alpha[j];
+        memset(wb, 0, cs * sizeof(double));
+        if (bj > 0.0) {
+            const double *qnb = Qb + cs * (j + 1);
+            double rq = 0.0;
+            for (size_t i = 0; i < cs; ++i) rq += qnb[i] * qn[i];
+            double btot = bb[j] - rq / bj;
+            for (size_t i = 0; i < cs; ++i) wb[i] = qnb[i] / bj + btot * qn[i];
+        }
+        /* w' = w - alpha q_j - beta_{j-1} q_{j-1} */
+        double r2 = 0.0, r3 = 0.0;
+        for (size_t i = 0; i < cs; ++i) { r2 += q[i] * wb[i]; r3 += qp[i] * wb[i]; }
+        double atot = ab[j] - r2;
+        if (j >= 2) bb[j - 1] += -r3;
+        double *qjb = Qb + cs * j, *qpb = Qb + cs * (j - 1);
+        for (size_t i = 0; i < cs; ++i) {
+            double wj = bj * qn[i] + aj * q[i] + bp * qp[i];     /* H q_j */
+            qjb[i] += -aj * wb[i] + atot * wj;
+            if (j >= 2) qpb[i] += -bp * wb[i];
+            wb[i] += atot * q[i];                               /* wbar */
+        }
+        orc_spmv(n, c->rowptr, c->col, c->val, c->g, diag, wb, Hw);
+        for (size_t i = 0; i < cs; ++i) qjb[i] += Hw[i];
+        for (int i = 0; i < n; ++i) dbar[i] += wb[2 * i] * q[2 * i] + wb[2 * i + 1] * q[2 * i + 1];
+        jv(c, wb, Hw);                                          /* J wbar */
+        double gs = 0.0;
+        for (size_t i = 0; i < cs; ++i) gs += Hw[i] * q[i];
+        *gbar += gs;
+    }
+    free(wb); free(Hw);
+}
+
+static int g_adjoint_mode = 1;   /* 0: exact CGS reverse, 1: three-term reverse */
+void orc_set_adjoint_mode(int mode) { g_adjoint_mode = mode; }
+
 /* reverse of one cycle: pbar (adjoint of the normalised output) ->
  * pbar (adjoint of the input); accumulates dbar[i] and *gbar */
 static void cycle_reverse(cycle_t *c, const double *diag, double *pbar, double *dbar, double *gbar)
@@ -401,6 +455,12 @@ static void cycle_reverse(cycle_t *c, const double *diag, double *pbar, double *
         ab[i + 1] = cim * Ab[2 * (i + i * nt) + 1];
         if (i + 1 < m)
             bb[i + 1] = cim * (Ab[2 * (i + (i + 1) * nt) + 1] + Ab[2 * ((i + 1) + i * nt) + 1]);
+    }
+    if (g_adjoint_mode == 1) {
+        lanczos_reverse_3term(c, diag, Qb, ab, bb, dbar, gbar);
+        memcpy(pbar, Qb + cs, cs * sizeof(double));
+        free(p1b); free(Qb); free(Eb); free(Ab); free(ab); free(bb);
+        return;
     }
     /* Lanczos, j = jb .. 1 */
     double *wb = (double *)malloc(cs * sizeof(double));

@@ -172,6 +172,11 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
                    const double *bra, double w_spec, const double *corr_target,
                    int ckpt, double *grad_e, double *grad_g);
 
+/* 0: reverse of the full-projection Lanczos as computed (O(m^2) vector ops);
+ * 1 (default): reverse of the equivalent three-term recurrence on the
+ * stored basis (O(m)), the form the engine uses. */
+void orc_set_adjoint_mode(int mode);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,8 +101,13 @@ def lib():
                                   C.c_uint64, C.c_int64, C.c_void_p, C.c_int, C.c_void_p,
                                   C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
                                   C.POINTER(C.c_double)]
+        L.orc_set_adjoint_mode.argtypes = [C.c_int]
         _lib = L
     return _lib
+
+
+def set_adjoint_mode(mode):
+    lib().orc_set_adjoint_mode(int(mode))
 
 
 def _ptr(a):

@@ -67,3 +67,21 @@ def test_taylor_remainder_is_second_order(oracle, small):
         rem.append(abs(L - L0 - t * ge @ v))
     assert rem[1] / rem[0] == pytest.approx(0.25, abs=0.05)
     assert rem[2] / rem[1] == pytest.approx(0.25, abs=0.05)
+
+
+def test_three_term_reverse_equals_full_projection_reverse(oracle, small):
+    """the engine's O(m) reverse (three-term recurrence on the stored basis)
+    against the reverse of the full-projection map as computed"""
+    s = small
+    n, T = s["n"], 25
+    p = oracle.make_params(n, 12, 2.0, sigma_static=200.0, seed=5, reorth=1)
+    sites = oracle.Sites(n, tau=50.0, sigma=300.0, e_site=0.0)
+    args = (s["rp"], s["col"], s["val"], s["psi0"], 1, T, s["grp"], 2, [0.3, 1.0])
+    try:
+        oracle.set_adjoint_mode(0)
+        a = oracle.adjoint(p, sites, *args, bra=s["bra"], w_spec=0.2)
+    finally:
+        oracle.set_adjoint_mode(1)
+    b = oracle.adjoint(p, sites, *args, bra=s["bra"], w_spec=0.2)
+    assert np.abs(a[1] - b[1]).max() < 1e-11 * np.abs(a[1]).max()
+    assert abs(a[2] - b[2]) < 1e-11 * abs(a[2])
