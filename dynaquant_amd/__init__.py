@@ -25,7 +25,7 @@ EXPORTS = [
     "dq_system_set_observables", "dq_tube_sites", "dq_tube_couplings", "dq_update_noise",
     "dq_propagate_step", "dq_run", "dq_get_step", "dq_get_populations", "dq_get_corr",
     "dq_get_error_sum", "dq_get_diagonal", "dq_spmv", "dq_pade_tbar", "dq_spectrum", "dq_fft",
-    "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times",
+    "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times", "dq_adjoint",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -222,6 +222,17 @@ class System:
         cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
         _check(lib().dq_get_kernel_times(self._h, _p(ms), _p(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+    def adjoint(self, nsteps, ckpt=50, w_pop=None, w_spec=0.0, corr_target=None):
+        """(loss[B], dL/d e_site [B, n], dL/dg [B]) by the reverse sweep"""
+        wp = None if w_pop is None else _c(w_pop, np.float64)
+        ct = None if corr_target is None else c2i(corr_target)
+        loss = np.zeros(self.B)
+        ge = np.zeros(self.B * self.n)
+        gg = np.zeros(self.B)
+        _check(lib().dq_adjoint(self._h, C.c_int64(nsteps), int(ckpt), _p(wp), C.c_double(w_spec),
+                                _p(ct), _p(loss), _p(ge), _p(gg)))
+        return loss, ge.reshape(self.B, self.n), gg
 
     def get_step(self):
         s = C.c_int64()

@@ -16,6 +16,7 @@
 
 #include "../../include/dynaquant.h"
 #include "dq_internal.h"
+#include "dq_adjoint.h"
 
 using dq::Dev;
 
@@ -481,12 +482,11 @@ static hipError_t collect_timings(dq_system* sys)
 
 extern "C" {
 
-static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
+static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
+                                    bool obs)
 {
-    const Dev& d = sys->d;
     hipStream_t st = sys->stream;
     hipError_t e;
-    const bool obs = (d.every > 0 || d.bra != nullptr);
     if ((e = timed(sys, DQ_K_PROLOGUE, [&] { return dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st); })) != hipSuccess) return e;
     if (first_in_step && obs && (e = timed(sys, DQ_K_NORM, [&] { return dq::launch_reduce_obs(d, st); })) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
@@ -499,6 +499,12 @@ static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in
     if ((e = timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); })) != hipSuccess) return e;
     if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_combine(d, st); })) != hipSuccess) return e;
     return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, last_in_step ? 1 : 0, st); });
+}
+
+static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
+{
+    const Dev& d = sys->d;
+    return enqueue_cycle_dev(sys, d, first_in_step, last_in_step, d.every > 0 || d.bra != nullptr);
 }
 
 static hipError_t enqueue_step(dq_system* sys)
@@ -787,6 +793,178 @@ dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft,
         spec[2 * k] *= scale;
         spec[2 * k + 1] *= scale;
     }
+    return DQ_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// adjoint
+// ---------------------------------------------------------------------------
+dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double* w_pop, double w_spec,
+                     const double* corr_target, double* loss, double* grad_e, double* grad_g)
+{
+    DQ_TRY(check_ready(sys));
+    Dev& d = sys->d;
+    if (d.plain || d.mixed) return fail(DQ_ERR_INVALID_ARG, "the adjoint runs on batched FP64 systems (force_batched)");
+    if (nsteps < 1 || ckpt < 1) return fail(DQ_ERR_INVALID_ARG, "nsteps and ckpt must be >= 1");
+    if (sys->step != 0) return fail(DQ_ERR_STATE, "the adjoint starts from step 0 (reset_disorder and set_state)");
+    if (!loss || !grad_e || !grad_g) return fail(DQ_ERR_INVALID_ARG, "null output");
+    if (sys->max_steps < nsteps) return fail(DQ_ERR_INVALID_ARG, "observables max_steps < nsteps");
+    const bool spec = sys->have_bra && w_spec != 0.0;
+    if (w_spec != 0.0 && !sys->have_bra) return fail(DQ_ERR_STATE, "spectral loss needs a bra");
+    if (w_pop && (sys->every <= 0 || sys->site_mode)) return fail(DQ_ERR_STATE, "population loss needs group observables");
+    hipStream_t st = sys->stream;
+    const int n = sys->n, Bp = d.Bp, m = sys->m, K = sys->K;
+    const size_t nB = (size_t)n * Bp;
+    ckpt = (int32_t)std::min<int64_t>(ckpt, nsteps);
+    const int ncyc = ckpt * K;
+    const int64_t nck = (nsteps + ckpt - 1) / ckpt;
+    const int max_s = 24;
+    std::vector<void*> tmp;
+    auto get = [&](void** p, size_t bytes) -> dq_status {
+        hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 16);
+        if (e != hipSuccess) return fail(DQ_ERR_OOM, std::string("adjoint hipMalloc: ") + hipGetErrorString(e));
+        tmp.push_back(*p);
+        hipMemsetAsync(*p, 0, bytes > 0 ? bytes : 16, st);
+        return DQ_OK;
+    };
+    auto cleanup = [&]() { hipStreamSynchronize(st); for (void* p : tmp) hipFree(p); };
+    dq::AdjDev a{};
+    a.m = m; a.K = K; a.nB = nB; a.Bp = Bp; a.max_s = max_s;
+    double2 *ck_psi, *ad_pbar;
+    double *ck_de, *ck_nrm;
+    dq_status rc = DQ_OK;
+#define AGET(p, bytes) do { if ((rc = get((void**)(p), (bytes))) != DQ_OK) { cleanup(); return rc; } } while (0)
+    AGET(&ck_psi, sizeof(double2) * nB * nck);
+    AGET(&ck_de, sizeof(double) * nB * nck);
+    AGET(&ck_nrm, sizeof(double) * Bp * nck);
+    AGET(&a.tapeV, sizeof(double2) * nB * (m + 1) * ncyc);
+    AGET(&a.sc, sizeof(double) * Bp * (m + 1) * ncyc);
+    AGET(&a.alpha, sizeof(double) * Bp * m * ncyc);
+    AGET(&a.beta, sizeof(double) * Bp * m * ncyc);
+    AGET(&a.nrm2, sizeof(double) * Bp * ncyc);
+    AGET(&a.tdiag, sizeof(double) * nB * ckpt);
+    AGET(&a.final_psi, sizeof(double2) * nB);
+    AGET(&a.pbar, sizeof(double2) * nB);
+    AGET(&a.pbar1, sizeof(double2) * nB);
+    AGET(&a.wb, sizeof(double2) * nB);
+    for (int q = 0; q < 3; ++q) AGET(&a.qb[q], sizeof(double2) * nB);
+    AGET(&a.grad_e, sizeof(double) * nB);
+    AGET(&a.gpart, sizeof(double) * Bp * d.ntile_s);
+    AGET(&a.y, sizeof(double2) * Bp * m);
+    AGET(&a.abar, sizeof(double) * Bp * m);
+    AGET(&a.bbar, sizeof(double) * Bp * m);
+    AGET(&a.r1, sizeof(double) * Bp);
+    AGET(&a.r2, sizeof(double) * Bp);
+    AGET(&a.r3, sizeof(double) * Bp * m);
+    AGET(&a.cbar, sizeof(double2) * Bp * (nsteps + 1));
+    a.pade_stride = (size_t)(16 + max_s + 1) * (dq::kMaxM + 2) * (dq::kMaxM + 2);
+    AGET(&a.pade_scratch, sizeof(double2) * a.pade_stride * Bp);
+    (void)ad_pbar;
+    // ---------------- forward with checkpoints ----------------
+    for (int64_t step = 0; step < nsteps; ++step) {
+        if (step % ckpt == 0) {
+            const int64_t k = step / ckpt;
+            HIP_TRY(hipMemcpyAsync(ck_psi + nB * k, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(ck_de + nB * k, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(ck_nrm + (size_t)Bp * k, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+        }
+        HIP_TRY(enqueue_step(sys));
+    }
+    HIP_TRY(enqueue_finalize(sys));
+    HIP_TRY(hipStreamSynchronize(st));
+    sys->step = nsteps;
+    DQ_TRY(dq_sync(sys));
+    // ---------------- loss and its adjoint at T (host) ----------------
+    std::vector<double2> hpsi(nB), hbar(nB, make_double2(0.0, 0.0));
+    HIP_TRY(hipMemcpy(hpsi.data(), d.psi, sizeof(double2) * nB, hipMemcpyDeviceToHost));
+    std::vector<int32_t> grp(n, -1);
+    if (w_pop) HIP_TRY(hipMemcpy(grp.data(), d.group, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    std::vector<double2> bra(n, make_double2(0.0, 0.0));
+    std::vector<double2> corr, cb((size_t)(nsteps + 1) * Bp, make_double2(0.0, 0.0));
+    if (spec) {
+        HIP_TRY(hipMemcpy(bra.data(), d.bra, sizeof(double2) * n, hipMemcpyDeviceToHost));
+        corr.resize((size_t)(nsteps + 1) * Bp);
+        HIP_TRY(hipMemcpy(corr.data(), d.corr, sizeof(double2) * corr.size(), hipMemcpyDeviceToHost));
+    }
+    for (int b = 0; b < sys->B; ++b) {
+        double L = 0.0;
+        if (w_pop)
+            for (int i = 0; i < n; ++i) {
+                const int g = grp[i];
+                if (g < 0) continue;
+                const double2 p = hpsi[(size_t)i * Bp + b];
+                L += w_pop[g] * (p.x * p.x + p.y * p.y);
+                hbar[(size_t)i * Bp + b] = make_double2(2.0 * w_pop[g] * p.x, 2.0 * w_pop[g] * p.y);
+            }
+        if (spec)
+            for (int64_t k = 0; k <= nsteps; ++k) {
+                const double2 c = corr[(size_t)k * Bp + b];
+                const double tr = corr_target ? corr_target[2 * k] : 0.0;
+                const double ti = corr_target ? corr_target[2 * k + 1] : 0.0;
+                const double dr = c.x - tr, di = c.y - ti;
+                L += w_spec * (dr * dr + di * di);
+                cb[(size_t)k * Bp + b] = make_double2(2.0 * w_spec * dr, 2.0 * w_spec * di);
+            }
+        loss[b] = L;
+    }
+    HIP_TRY(hipMemcpyAsync(a.pbar, hbar.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(a.cbar, cb.data(), sizeof(double2) * cb.size(), hipMemcpyHostToDevice, st));
+    if (spec) HIP_TRY(dq::adj_add_corr(d, a, nsteps, a.pbar, st));
+    // ---------------- reverse over segments ----------------
+    for (int64_t seg = nck - 1; seg >= 0; --seg) {
+        const int64_t s0 = seg * ckpt, s1 = std::min<int64_t>(s0 + ckpt, nsteps);
+        const int nc = (int)(s1 - s0) * K;
+        HIP_TRY(hipMemcpyAsync(d.psi, ck_psi + nB * seg, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d.de, ck_de + nB * seg, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d.nrm2, ck_nrm + (size_t)Bp * seg, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+        int64_t s0v = s0;
+        HIP_TRY(hipMemcpyAsync(d.step, &s0v, sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int c = 0; c < nc; ++c) {
+            const int k = c % K;
+            Dev dc = d;
+            dc.V = a.tapeV + (size_t)c * (m + 1) * nB;
+            dc.sc = a.sc + (size_t)c * (m + 1) * Bp;
+            dc.alpha = a.alpha + (size_t)c * m * Bp;
+            dc.beta = a.beta + (size_t)c * m * Bp;
+            dc.diag = a.tdiag + (size_t)(c / K) * nB;
+            HIP_TRY(enqueue_cycle_dev(sys, dc, k == 0, k == K - 1, false));
+            HIP_TRY(hipMemcpyAsync(a.nrm2 + (size_t)c * Bp, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+        }
+        {
+            Dev dc = d;
+            dc.V = a.final_psi;
+            HIP_TRY(dq::launch_prologue(dc, 1, 0, 0, 1, st));
+        }
+        a.last_c = nc - 1;
+        for (int c = nc - 1; c >= 0; --c) {
+            HIP_TRY(dq::adj_renorm(d, a, c, st));
+            HIP_TRY(dq::adj_pade(d, a, c, st));
+            HIP_TRY(dq::adj_lanczos(d, a, c, st));
+            HIP_TRY(hipMemcpyAsync(a.pbar, a.qbar_slot(0), sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
+            if (c % K == 0 && spec) HIP_TRY(dq::adj_add_corr(d, a, s0 + c / K, a.pbar, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    DQ_TRY(dq_sync(sys));
+    // ---------------- gradients ----------------
+    std::vector<double> ge(nB), gp((size_t)Bp * d.ntile_s);
+    HIP_TRY(hipMemcpy(ge.data(), a.grad_e, sizeof(double) * nB, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(gp.data(), a.gpart, sizeof(double) * gp.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b) {
+        for (int i = 0; i < n; ++i) grad_e[(size_t)b * n + i] = ge[(size_t)i * Bp + b];
+        double g = 0.0;
+        for (int t = 0; t < d.ntile_s; ++t) g += gp[(size_t)t * Bp + b];
+        grad_g[b] = g;
+    }
+    // leave the system at the final state of the forward run
+    HIP_TRY(hipMemcpyAsync(d.psi, hpsi.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, st));
+    int64_t sT = nsteps;
+    HIP_TRY(hipMemcpyAsync(d.step, &sT, sizeof(int64_t), hipMemcpyHostToDevice, st));
+    HIP_TRY(dq::launch_set_unit_norm(d, st));
+    cleanup();
+#undef AGET
     return DQ_OK;
 }
 

@@ -177,6 +177,21 @@ dq_status dq_spectrum(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
  * two, <= 65536) on the GPU (replaces cufftExecZ2Z, spectraClass.cu:617). */
 dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
 
+/* ---- adjoint ---------------------------------------------------------- */
+/* Gradient of the per-trajectory loss
+ *   L_b = sum_g w_pop[g] P_{b,g}(T) + w_spec sum_{n=0..T} |C_b(n) - C_target(n)|^2
+ * (groups and bra from dq_system_set_observables; w_pop or C_target may be
+ * NULL) over nsteps steps from the current state at step 0, with respect
+ * to the site energies (grad_e[b][i]) and the coupling scale (grad_g[b]).
+ * Exact reverse sweep of the discrete map: states are checkpointed every
+ * `ckpt` steps, each segment is recomputed with the noise replayed from
+ * its Philox stream, and the Lanczos part is reversed through its
+ * three-term recurrence.  Batched FP64 systems only.  No counterpart in
+ * the reference (it has no derivatives). */
+dq_status dq_adjoint(dq_system *sys, int64_t nsteps, int32_t ckpt, const double *w_pop,
+                     double w_spec, const double *corr_target, double *loss, double *grad_e,
+                     double *grad_g);
+
 /* ---- timing ----------------------------------------------------------- */
 /* Milliseconds of the last dq_run, measured with HIP events on the
  * system's stream. */
