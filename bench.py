@@ -110,7 +110,7 @@ def main():
     print("# workload N=%d nnz_off=%d B=%d m=%d dt=%g" % (n, nnz, B, M, DT), file=sys.stderr, flush=True)
 
     # ---- headline: batched mixed precision, hipGraph per step
-    s = make_system(dq, t, B, dq.MIXED, W + K)
+    s = make_system(dq, t, B, dq.MIXED, W + K + args.profile_steps)
     s.run(W, graph=True)
     s.run(K, graph=True)
     ms = s.last_run_ms()

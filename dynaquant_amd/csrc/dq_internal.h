@@ -78,6 +78,11 @@ hipError_t launch_spmm(const Dev& d, int j, int last, hipStream_t s);
 hipError_t launch_scalars_h(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update(const Dev& d, int j, hipStream_t s);
 hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s);
+// batched variants (dq_batched.hip)
+hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s);
+hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s);
+hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
+hipError_t launch_combine_b(const Dev& d, hipStream_t s);
 hipError_t launch_pade(const Dev& d, hipStream_t s);
 hipError_t launch_combine(const Dev& d, hipStream_t s);
 hipError_t launch_finish_norm(const Dev& d, int advance_step, hipStream_t s);

@@ -304,81 +304,8 @@ hipError_t launch_reduce_obs(const Dev& d, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Batched SpMM: w[i][b] = d[i][b] x[i][b] + g * sum_p val[p] x[col[p]][b]
-// (cusparseZcsrmv per trajectory in the reference, lanczosClass.cu:618-661).
-// One wave = one row x 64 trajectories; the row's (col, val) stream is
-// wave-uniform (scalar loads) and every nonzero gathers 64 contiguous
-// trajectory entries of x.  The four waves of a block take rows
-// r0+w, r0+w+4, ...; tiles are mapped XCD-contiguously.
-// last == 0: store w into slot j+1, partial <v_j|w>;  last == 1: partial |w|^2.
+// SpMV of the single-trajectory path; the batched SpMM is in dq_batched.hip.
 // ---------------------------------------------------------------------------
-template <typename VT>
-__global__ __launch_bounds__(kThreads) void k_spmm(Dev d, int j, int last)
-{
-    using V2 = typename Vec2<VT>::T;
-    const int ntt = d.Bp / 64;
-    const int nblk = d.ntile_s * ntt;
-    const int logical = xcd_tile(blockIdx.x, gridDim.x);
-    if (logical >= nblk) return;
-    const int rtile = logical / ntt, ttile = logical % ntt;
-    const int lane = threadIdx.x & 63;
-    const int wave = uniform(threadIdx.x >> 6);
-    const int b = ttile * 64 + lane;
-    const size_t nB = (size_t)d.n * d.Bp;
-    const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB;
-    V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
-    const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
-    const VT* __restrict__ val;
-    if constexpr (sizeof(VT) == 4) val = d.val32;
-    else val = d.val64;
-    const int nact = d.nact[b];
-    const VT g = (VT)d.g;
-    double acc0 = 0.0, acc1 = 0.0;
-    const int r0 = rtile * d.rt_s, r1 = min(d.n, r0 + d.rt_s);
-    for (int i = r0 + wave; i < r1; i += 4) {
-        const int p0 = d.rowptr[i], p1 = d.rowptr[i + 1];
-        VT sr = 0, si = 0;
-        int p = p0;
-        for (; p + 4 <= p1; p += 4) {
-            const int c0 = d.col[p], c1 = d.col[p + 1], c2 = d.col[p + 2], c3 = d.col[p + 3];
-            const VT a0 = val[p], a1 = val[p + 1], a2 = val[p + 2], a3 = val[p + 3];
-            const V2 x0 = x[(size_t)c0 * d.Bp + b];
-            const V2 x1 = x[(size_t)c1 * d.Bp + b];
-            const V2 x2 = x[(size_t)c2 * d.Bp + b];
-            const V2 x3 = x[(size_t)c3 * d.Bp + b];
-            sr += a0 * x0.x; si += a0 * x0.y;
-            sr += a1 * x1.x; si += a1 * x1.y;
-            sr += a2 * x2.x; si += a2 * x2.y;
-            sr += a3 * x3.x; si += a3 * x3.y;
-        }
-        for (; p < p1; ++p) {
-            const int c0 = d.col[p];
-            const VT a0 = val[p];
-            const V2 x0 = x[(size_t)c0 * d.Bp + b];
-            sr += a0 * x0.x; si += a0 * x0.y;
-        }
-        const size_t e = (size_t)i * d.Bp + b;
-        const V2 xi = x[e];
-        const VT di = diag[e];
-        VT wr = di * xi.x + g * sr;
-        VT wi = di * xi.y + g * si;
-        if (i >= nact) { wr = 0; wi = 0; }
-        if (!last) {
-            V2 w;
-            w.x = wr;
-            w.y = wi;
-            y[e] = w;
-            const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
-            acc0 += xr * dr + xm * dm;
-            acc1 += xr * dm - xm * dr;
-        } else {
-            acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
-        }
-    }
-    double out[2] = {acc0, acc1};
-    store_partials<false, 2>(out, last ? 1 : 2, d.part, d.ntile_s, rtile, d.Bp, b);
-}
-
 // Single-trajectory CSR SpMV (FP64): 16 lanes per row, 64 rows per block.
 __global__ __launch_bounds__(kThreads) void k_spmv_plain(Dev d, int j, int last)
 {
@@ -424,13 +351,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_plain(Dev d, int j, int last)
 
 hipError_t launch_spmm(const Dev& d, int j, int last, hipStream_t s)
 {
-    if (d.plain) {
-        hipLaunchKernelGGL(k_spmv_plain, dim3(pad8(d.ntile_s)), dim3(kThreads), 0, s, d, j, last);
-    } else {
-        dim3 grid(pad8(d.ntile_s * (d.Bp / 64)));
-        if (d.mixed) hipLaunchKernelGGL((k_spmm<float>), grid, dim3(kThreads), 0, s, d, j, last);
-        else hipLaunchKernelGGL((k_spmm<double>), grid, dim3(kThreads), 0, s, d, j, last);
-    }
+    if (!d.plain) return launch_spmm_b(d, j, last, s);
+    hipLaunchKernelGGL(k_spmv_plain, dim3(pad8(d.ntile_s)), dim3(kThreads), 0, s, d, j, last);
     return hipGetLastError();
 }
 
@@ -515,8 +437,8 @@ __global__ __launch_bounds__(kThreads) void k_scalars_h(Dev d, int j)
 
 hipError_t launch_scalars_h(const Dev& d, int j, hipStream_t s)
 {
-    if (d.plain) hipLaunchKernelGGL((k_scalars_h<true>), dim3(1), dim3(kThreads), 0, s, d, j);
-    else hipLaunchKernelGGL((k_scalars_h<false>), dim3(d.Bp / 64), dim3(kThreads), 0, s, d, j);
+    if (!d.plain) return launch_scalars_hb(d, j, s);
+    hipLaunchKernelGGL((k_scalars_h<true>), dim3(1), dim3(kThreads), 0, s, d, j);
     return hipGetLastError();
 }
 
@@ -615,13 +537,8 @@ static void launch_update_t(const Dev& d, int j, hipStream_t s)
 
 hipError_t launch_update(const Dev& d, int j, hipStream_t s)
 {
-    if (d.plain) {
-        if (d.mixed) launch_update_t<true, float>(d, j, s);
-        else launch_update_t<true, double>(d, j, s);
-    } else {
-        if (d.mixed) launch_update_t<false, float>(d, j, s);
-        else launch_update_t<false, double>(d, j, s);
-    }
+    if (!d.plain) return launch_update_b(d, j, s);
+    launch_update_t<true, double>(d, j, s);
     return hipGetLastError();
 }
 
@@ -997,13 +914,8 @@ static void launch_combine_t(const Dev& d, hipStream_t s)
 
 hipError_t launch_combine(const Dev& d, hipStream_t s)
 {
-    if (d.plain) {
-        if (d.mixed) launch_combine_t<true, float>(d, s);
-        else launch_combine_t<true, double>(d, s);
-    } else {
-        if (d.mixed) launch_combine_t<false, float>(d, s);
-        else launch_combine_t<false, double>(d, s);
-    }
+    if (!d.plain) return launch_combine_b(d, s);
+    launch_combine_t<true, double>(d, s);
     return hipGetLastError();
 }
 
