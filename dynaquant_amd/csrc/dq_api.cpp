@@ -8,6 +8,7 @@
 // SpMV; here the host only enqueues.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -152,7 +153,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.mixed = cfg->precision == DQ_PREC_MIXED;
     d.plain = plain;
     d.rt_e = plain ? 1024 : 128;
-    d.rt_s = 64;
+    d.rt_s = plain ? 64 : 8 * dq::kRB;   // batched: 4 waves x 2 row blocks
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.S = cfg->ou_substeps;
@@ -279,7 +280,48 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
         HIP_TRY(hipMemcpyAsync(v32, f32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, sys->stream));
     }
     HIP_TRY(hipMemcpyAsync(es, esv.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    // row-blocked copy: per block of kRB rows the sorted union of columns and
+    // kRB values per union column
+    const int R = dq::kRB, nblk = (n + R - 1) / R;
+    std::vector<int32_t> rbp(nblk + 1, 0), rbc;
+    std::vector<double> rbv64;
+    for (int k = 0; k < nblk; ++k) {
+        const int r0 = k * R, r1 = std::min(n, r0 + R);
+        std::vector<int32_t> u;
+        for (int i = r0; i < r1; ++i) u.insert(u.end(), col + rowptr[i], col + rowptr[i + 1]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        const size_t base = rbc.size();
+        rbc.insert(rbc.end(), u.begin(), u.end());
+        rbv64.resize(rbc.size() * R, 0.0);
+        for (int i = r0; i < r1; ++i)
+            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
+                rbv64[(base + pos) * R + (i - r0)] = val[p];
+            }
+        rbp[k + 1] = (int32_t)rbc.size();
+    }
+    std::vector<float> rbv32(rbv64.size());
+    for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
+    sys->release(*(int32_t**)&d.rb_ptr);
+    sys->release(*(int32_t**)&d.rb_col);
+    sys->release(*(float**)&d.rb_val32);
+    sys->release(*(double**)&d.rb_val64);
+    int32_t *drbp, *drbc;
+    float* drv32;
+    double* drv64;
+    DQ_TRY(sys->alloc(&drbp, rbp.size()));
+    DQ_TRY(sys->alloc(&drbc, std::max<size_t>(rbc.size(), 1)));
+    DQ_TRY(sys->alloc(&drv32, std::max<size_t>(rbv32.size(), 1)));
+    DQ_TRY(sys->alloc(&drv64, std::max<size_t>(rbv64.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(drbp, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
+    if (!rbc.empty()) {
+        HIP_TRY(hipMemcpyAsync(drbc, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(drv32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(drv64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
+    }
     HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.rb_ptr = drbp; d.rb_col = drbc; d.rb_val32 = drv32; d.rb_val64 = drv64;
     d.rowptr = rp; d.col = cl; d.val64 = v64; d.val32 = v32; d.e_site = es;
     sys->have_h = true;
     return DQ_OK;

@@ -1,10 +1,7 @@
 // dq_batched.hip -- batched (B >= 64 padded) kernels of the Krylov cycle.
 //
-// SpMM: one wave per row x 64 trajectories.  The row's column indices and
-// values are fetched lane-parallel (one coalesced load each for up to 64
-// nonzeros) and broadcast with readlane, so the x gathers of a row issue
-// back to back, eight at a time, with no scalar-load round trips between
-// them; every gather is one contiguous 64-trajectory run of x.
+// SpMM: one wave per block of 16 rows x 64 trajectories over the union of
+// the rows' columns (register reuse of every gathered x run).
 //
 // Update and combine: a wave covers 16 trajectories x 4 groups of Krylov
 // vectors (lane = 16 * kgroup + t).  Each lane streams only the vectors of
@@ -64,7 +61,13 @@ __device__ __forceinline__ void bstore_partials(const double (&acc)[NQ], int nq,
 }
 
 // ---------------------------------------------------------------------------
-// SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory)
+// SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory).
+// Row-blocked: a wave owns kRB consecutive rows x 64 trajectories and walks
+// the union of their columns, so each gathered x run (64 trajectories, one
+// contiguous 512 B / 1 KB segment) feeds kRB rows from registers.  For the
+// dipole tubes kRB = 16 cuts the gathers per row from ~61 to ~10.  Column
+// indices and the kRB values per column are fetched lane-parallel and
+// broadcast with readlane.
 // ---------------------------------------------------------------------------
 template <typename VT>
 __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
@@ -82,69 +85,70 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
     const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB;
     V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
     const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
-    const VT* __restrict__ val;
-    if constexpr (sizeof(VT) == 4) val = d.val32;
-    else val = d.val64;
+    const VT* __restrict__ rv;
+    if constexpr (sizeof(VT) == 4) rv = d.rb_val32;
+    else rv = d.rb_val64;
     const int nact = d.nact[b];
     const VT g = (VT)d.g;
     double acc0 = 0.0, acc1 = 0.0;
-    const int r0 = rtile * d.rt_s, r1 = min(d.n, r0 + d.rt_s);
-    for (int i = r0 + wave; i < r1; i += 4) {
-        const int p0 = uniform(d.rowptr[i]);
-        const int cnt = uniform(d.rowptr[i + 1]) - p0;
-        VT sr0 = 0, si0 = 0, sr1 = 0, si1 = 0;
-        for (int base = 0; base < cnt; base += 64) {
-            const int lim = min(64, cnt - base);
-            int mc = 0;
-            VT mv = 0;
-            if (lane < lim) {
-                mc = d.col[p0 + base + lane];
-                mv = val[p0 + base + lane];
-            }
-            int q = 0;
-            for (; q + 8 <= lim; q += 8) {
-                int c[8];
-                VT v[8];
+    const int nblk = (d.n + kRB - 1) / kRB;
+    for (int rb = rtile * (d.rt_s / kRB) + wave; rb < min(nblk, (rtile + 1) * (d.rt_s / kRB)); rb += 4) {
+        const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+        VT ar[kRB], ai[kRB];
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { ar[r] = 0; ai[r] = 0; }
+        for (int cb = u0; cb < u1; cb += 64) {
+            const int lim = min(64, u1 - cb);
+            const int mc = lane < lim ? d.rb_col[cb + lane] : 0;
+            for (int q = 0; q < lim; q += 8) {
+                // values of columns q..q+7 of this chunk: two lane-parallel loads
+                const size_t vb = (size_t)(cb + q) * kRB;
+                const size_t vend = (size_t)u1 * kRB;
+                const VT mv0 = vb + lane < vend ? rv[vb + lane] : (VT)0;
+                const VT mv1 = vb + 64 + lane < vend ? rv[vb + 64 + lane] : (VT)0;
                 V2 xv[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    c[u] = __builtin_amdgcn_readlane(mc, q + u);
-                    v[u] = bcast(mv, q + u);
+                    if (q + u < lim) {
+                        const int c = __builtin_amdgcn_readlane(mc, q + u);
+                        xv[u] = x[(size_t)c * Bp + b];
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) xv[u] = x[(size_t)c[u] * Bp + b];
+                for (int u = 0; u < 8; ++u) {
+                    if (q + u < lim) {
 #pragma unroll
-                for (int u = 0; u < 8; u += 2) {
-                    sr0 += v[u] * xv[u].x;
-                    si0 += v[u] * xv[u].y;
-                    sr1 += v[u + 1] * xv[u + 1].x;
-                    si1 += v[u + 1] * xv[u + 1].y;
+                        for (int r = 0; r < kRB; ++r) {
+                            const VT v = u < 4 ? bcast(mv0, u * kRB + r) : bcast(mv1, (u - 4) * kRB + r);
+                            ar[r] += v * xv[u].x;
+                            ai[r] += v * xv[u].y;
+                        }
+                    }
                 }
-            }
-            for (; q < lim; ++q) {
-                const int c0 = __builtin_amdgcn_readlane(mc, q);
-                const VT v0 = bcast(mv, q);
-                const V2 x0 = x[(size_t)c0 * Bp + b];
-                sr0 += v0 * x0.x;
-                si0 += v0 * x0.y;
             }
         }
-        const size_t e = (size_t)i * Bp + b;
-        const V2 xi = x[e];
-        const VT di = diag[e];
-        VT wr = di * xi.x + g * (sr0 + sr1);
-        VT wi = di * xi.y + g * (si0 + si1);
-        if (i >= nact) { wr = 0; wi = 0; }
-        if (!last) {
-            V2 w;
-            w.x = wr;
-            w.y = wi;
-            y[e] = w;
-            const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
-            acc0 += xr * dr + xm * dm;
-            acc1 += xr * dm - xm * dr;
-        } else {
-            acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) {
+            const int i = rb * kRB + r;
+            if (i < d.n) {
+                const size_t e = (size_t)i * Bp + b;
+                const V2 xi = x[e];
+                const VT di = diag[e];
+                VT wr = di * xi.x + g * ar[r];
+                VT wi = di * xi.y + g * ai[r];
+                if (i >= nact) { wr = 0; wi = 0; }
+                if (!last) {
+                    V2 w;
+                    w.x = wr;
+                    w.y = wi;
+                    y[e] = w;
+                    const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
+                    acc0 += xr * dr + xm * dm;
+                    acc1 += xr * dm - xm * dr;
+                } else {
+                    acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
+                }
+            }
         }
     }
     double out[2] = {acc0, acc1};
@@ -173,11 +177,20 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
-    double c0 = 0.0, c1 = 0.0;
-    for (int t = wave; t < d.ntile_s; t += 4) {
-        c0 += d.part[(size_t)t * Bp + b];
-        c1 += d.part[((size_t)d.ntile_s + t) * Bp + b];
+    // four independent chains per lane keep the tile loads in flight
+    double c0a = 0.0, c0b = 0.0, c1a = 0.0, c1b = 0.0;
+    int t = wave;
+    for (; t + 4 < d.ntile_s; t += 8) {
+        c0a += d.part[(size_t)t * Bp + b];
+        c1a += d.part[((size_t)d.ntile_s + t) * Bp + b];
+        c0b += d.part[(size_t)(t + 4) * Bp + b];
+        c1b += d.part[((size_t)d.ntile_s + t + 4) * Bp + b];
     }
+    if (t < d.ntile_s) {
+        c0a += d.part[(size_t)t * Bp + b];
+        c1a += d.part[((size_t)d.ntile_s + t) * Bp + b];
+    }
+    const double c0 = c0a + c0b, c1 = c1a + c1b;
     lds[wave][lane] = c0;
     __syncthreads();
     const double cr = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);

@@ -14,6 +14,7 @@ namespace dq {
 constexpr int kMaxM = 32;          // Krylov depth limit (Pade matrix <= 34)
 constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
+constexpr int kRB = 16;            // rows per block of the row-blocked SpMM
 
 struct Dev {
     int n, Bp, B, m;
@@ -32,6 +33,12 @@ struct Dev {
     const int32_t* col;
     const double* val64;
     const float* val32;
+    // row-blocked copy for the batched SpMM: blocks of kRB rows, the union
+    // of their columns and kRB values per union column (0 where absent)
+    const int32_t* rb_ptr;          // [n/kRB + 1] offsets into rb_col
+    const int32_t* rb_col;          // [total union columns]
+    const float* rb_val32;          // [total][kRB]
+    const double* rb_val64;         // [total][kRB]
     const double* inv_tau;          // [n]
     const double* coeff;            // [n] sigma*sqrt(2/tau)
     const double* ou_sigma;         // [n]

@@ -102,7 +102,14 @@ __device__ __forceinline__ double block_tile_sum(const double* part, int q, int 
         __syncthreads();
         s = (lds[0][0] + lds[1][0]) + (lds[2][0] + lds[3][0]);
     } else {
-        for (int t = wave; t < ntile; t += 4) s += part[((size_t)q * ntile + t) * Bp + b];
+        double s2 = 0.0;
+        int t = wave;
+        for (; t + 4 < ntile; t += 8) {
+            s += part[((size_t)q * ntile + t) * Bp + b];
+            s2 += part[((size_t)q * ntile + t + 4) * Bp + b];
+        }
+        if (t < ntile) s += part[((size_t)q * ntile + t) * Bp + b];
+        s += s2;
         lds[wave][lane] = s;
         __syncthreads();
         s = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
