@@ -73,10 +73,11 @@ template <typename VT>
 __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
 {
     using V2 = typename BVec2<VT>::T;
+    __shared__ VT s_ar[4][kRB][64], s_ai[4][kRB][64];
     const int ntt = d.Bp / 64;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_s * ntt) return;
-    const int rtile = logical / ntt, ttile = logical % ntt;
+    const int rb = logical / ntt, ttile = logical % ntt;     // one row block per block
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int b = ttile * 64 + lane;
@@ -88,71 +89,78 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
     const VT* __restrict__ rv;
     if constexpr (sizeof(VT) == 4) rv = d.rb_val32;
     else rv = d.rb_val64;
-    const int nact = d.nact[b];
-    const VT g = (VT)d.g;
-    double acc0 = 0.0, acc1 = 0.0;
-    const int nblk = (d.n + kRB - 1) / kRB;
-    for (int rb = rtile * (d.rt_s / kRB) + wave; rb < min(nblk, (rtile + 1) * (d.rt_s / kRB)); rb += 4) {
-        const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
-        VT ar[kRB], ai[kRB];
+    // the four waves take contiguous quarters (multiples of 8) of the union
+    const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+    const int len = u1 - u0;
+    const int quarter = ((len + 31) / 32) * 8;
+    const int w0 = min(u1, u0 + wave * quarter), w1 = min(u1, w0 + quarter);
+    VT ar[kRB], ai[kRB];
 #pragma unroll
-        for (int r = 0; r < kRB; ++r) { ar[r] = 0; ai[r] = 0; }
-        for (int cb = u0; cb < u1; cb += 64) {
-            const int lim = min(64, u1 - cb);
-            const int mc = lane < lim ? d.rb_col[cb + lane] : 0;
-            for (int q = 0; q < lim; q += 8) {
-                // values of columns q..q+7 of this chunk: two lane-parallel loads
-                const size_t vb = (size_t)(cb + q) * kRB;
-                const size_t vend = (size_t)u1 * kRB;
-                const VT mv0 = vb + lane < vend ? rv[vb + lane] : (VT)0;
-                const VT mv1 = vb + 64 + lane < vend ? rv[vb + 64 + lane] : (VT)0;
-                V2 xv[8];
+    for (int r = 0; r < kRB; ++r) { ar[r] = 0; ai[r] = 0; }
+    for (int cb = w0; cb < w1; cb += 64) {
+        const int lim = min(64, w1 - cb);
+        const int mc = lane < lim ? d.rb_col[cb + lane] : 0;
+        for (int q = 0; q < lim; q += 8) {
+            const size_t vb = (size_t)(cb + q) * kRB;
+            const size_t vend = (size_t)w1 * kRB;
+            const VT mv0 = vb + lane < vend ? rv[vb + lane] : (VT)0;
+            const VT mv1 = vb + 64 + lane < vend ? rv[vb + 64 + lane] : (VT)0;
+            V2 xv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (q + u < lim) {
-                        const int c = __builtin_amdgcn_readlane(mc, q + u);
-                        xv[u] = x[(size_t)c * Bp + b];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (q + u < lim) {
-#pragma unroll
-                        for (int r = 0; r < kRB; ++r) {
-                            const VT v = u < 4 ? bcast(mv0, u * kRB + r) : bcast(mv1, (u - 4) * kRB + r);
-                            ar[r] += v * xv[u].x;
-                            ai[r] += v * xv[u].y;
-                        }
-                    }
+            for (int u = 0; u < 8; ++u) {
+                if (q + u < lim) {
+                    const int c = __builtin_amdgcn_readlane(mc, q + u);
+                    xv[u] = x[(size_t)c * Bp + b];
                 }
             }
-        }
 #pragma unroll
-        for (int r = 0; r < kRB; ++r) {
-            const int i = rb * kRB + r;
-            if (i < d.n) {
-                const size_t e = (size_t)i * Bp + b;
-                const V2 xi = x[e];
-                const VT di = diag[e];
-                VT wr = di * xi.x + g * ar[r];
-                VT wi = di * xi.y + g * ai[r];
-                if (i >= nact) { wr = 0; wi = 0; }
-                if (!last) {
-                    V2 w;
-                    w.x = wr;
-                    w.y = wi;
-                    y[e] = w;
-                    const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
-                    acc0 += xr * dr + xm * dm;
-                    acc1 += xr * dm - xm * dr;
-                } else {
-                    acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
+            for (int u = 0; u < 8; ++u) {
+                if (q + u < lim) {
+#pragma unroll
+                    for (int r = 0; r < kRB; ++r) {
+                        const VT v = u < 4 ? bcast(mv0, u * kRB + r) : bcast(mv1, (u - 4) * kRB + r);
+                        ar[r] += v * xv[u].x;
+                        ai[r] += v * xv[u].y;
+                    }
                 }
             }
         }
     }
+#pragma unroll
+    for (int r = 0; r < kRB; ++r) { s_ar[wave][r][lane] = ar[r]; s_ai[wave][r][lane] = ai[r]; }
+    __syncthreads();
+    // epilogue: wave w finishes rows w, w+4, ... of the block
+    const int nact = d.nact[b];
+    const VT g = (VT)d.g;
+    double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < kRB / 4; ++rr) {
+        const int r = wave + 4 * rr;
+        const int i = rb * kRB + r;
+        if (i < d.n) {
+            const VT sr = (s_ar[0][r][lane] + s_ar[1][r][lane]) + (s_ar[2][r][lane] + s_ar[3][r][lane]);
+            const VT si = (s_ai[0][r][lane] + s_ai[1][r][lane]) + (s_ai[2][r][lane] + s_ai[3][r][lane]);
+            const size_t e = (size_t)i * Bp + b;
+            const V2 xi = x[e];
+            const VT di = diag[e];
+            VT wr = di * xi.x + g * sr;
+            VT wi = di * xi.y + g * si;
+            if (i >= nact) { wr = 0; wi = 0; }
+            if (!last) {
+                V2 w;
+                w.x = wr;
+                w.y = wi;
+                y[e] = w;
+                const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
+                acc0 += xr * dr + xm * dm;
+                acc1 += xr * dm - xm * dr;
+            } else {
+                acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
+            }
+        }
+    }
     double out[2] = {acc0, acc1};
-    bstore_partials<2>(out, last ? 1 : 2, d.part, d.ntile_s, rtile, Bp, b);
+    bstore_partials<2>(out, last ? 1 : 2, d.part, d.ntile_s, rb, Bp, b);
 }
 
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
@@ -178,19 +186,21 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     const int b = blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
     // four independent chains per lane keep the tile loads in flight
-    double c0a = 0.0, c0b = 0.0, c1a = 0.0, c1b = 0.0;
+    double c0v[4] = {0.0, 0.0, 0.0, 0.0}, c1v[4] = {0.0, 0.0, 0.0, 0.0};
     int t = wave;
-    for (; t + 4 < d.ntile_s; t += 8) {
-        c0a += d.part[(size_t)t * Bp + b];
-        c1a += d.part[((size_t)d.ntile_s + t) * Bp + b];
-        c0b += d.part[(size_t)(t + 4) * Bp + b];
-        c1b += d.part[((size_t)d.ntile_s + t + 4) * Bp + b];
+    for (; t + 12 < d.ntile_s; t += 16) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            c0v[u] += d.part[(size_t)(t + 4 * u) * Bp + b];
+            c1v[u] += d.part[((size_t)d.ntile_s + t + 4 * u) * Bp + b];
+        }
     }
-    if (t < d.ntile_s) {
-        c0a += d.part[(size_t)t * Bp + b];
-        c1a += d.part[((size_t)d.ntile_s + t) * Bp + b];
+    for (; t < d.ntile_s; t += 4) {
+        c0v[0] += d.part[(size_t)t * Bp + b];
+        c1v[0] += d.part[((size_t)d.ntile_s + t) * Bp + b];
     }
-    const double c0 = c0a + c0b, c1 = c1a + c1b;
+    const double c0 = (c0v[0] + c0v[1]) + (c0v[2] + c0v[3]);
+    const double c1 = (c1v[0] + c1v[1]) + (c1v[2] + c1v[3]);
     lds[wave][lane] = c0;
     __syncthreads();
     const double cr = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
@@ -303,17 +313,34 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
 #pragma unroll
     for (int q = 0; q < 2 * KW + 1; ++q) acc[q] = 0.0;
     const int r0 = rtile * d.rt_e, r1 = min(d.n, r0 + d.rt_e);
+    // software pipeline: the vectors of row i+4 load while row i is combined
+    V2 nk[KW], nw;
+    if (r0 + wave < r1) {
+        const size_t e = (size_t)(r0 + wave) * Bp + b;
+#pragma unroll
+        for (int u = 0; u < KW; ++u) {
+            const int k = kg + 4 * u;
+            if (k <= j) nk[u] = V[(size_t)k * nB + e];
+        }
+        if (kg == 0) nw = out[e];
+    }
     for (int i = r0 + wave; i < r1; i += 4) {
         const size_t e = (size_t)i * Bp + b;
         V2 vk[KW];
 #pragma unroll
-        for (int u = 0; u < KW; ++u) {
-            const int k = kg + 4 * u;
-            if (k <= j) vk[u] = V[(size_t)k * nB + e];
+        for (int u = 0; u < KW; ++u) vk[u] = nk[u];
+        const V2 w = nw;
+        if (i + 4 < r1) {
+            const size_t en = e + (size_t)4 * Bp;
+#pragma unroll
+            for (int u = 0; u < KW; ++u) {
+                const int k = kg + 4 * u;
+                if (k <= j) nk[u] = V[(size_t)k * nB + en];
+            }
+            if (kg == 0) nw = out[en];
         }
         double pr = 0.0, pi = 0.0;
         if (kg == 0) {
-            const V2 w = out[e];
             pr = sw * (double)w.x;
             pi = sw * (double)w.y;
         }
@@ -421,19 +448,36 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
     const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
     double acc = 0.0;
     const int r0 = rtile * d.rt_e, r1 = min(d.n, r0 + d.rt_e);
-    for (int i = r0 + wave; i < r1; i += 4) {
-        const size_t e = (size_t)i * Bp + b;
-        double pr = 0.0, pi = 0.0;
-        if (kg == 0) {
-            const double2 p = d.psi[e];
-            pr = y[0].x * p.x - y[0].y * p.y;
-            pi = y[0].x * p.y + y[0].y * p.x;
-        }
-        V2 vk[KW];
+    V2 nk[KW];
+    double2 np = make_double2(0.0, 0.0);
+    if (r0 + wave < r1) {
+        const size_t e = (size_t)(r0 + wave) * Bp + b;
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k >= 1 && k < m) vk[u] = V[(size_t)k * nB + e];
+            if (k >= 1 && k < m) nk[u] = V[(size_t)k * nB + e];
+        }
+        if (kg == 0) np = d.psi[e];
+    }
+    for (int i = r0 + wave; i < r1; i += 4) {
+        const size_t e = (size_t)i * Bp + b;
+        V2 vk[KW];
+#pragma unroll
+        for (int u = 0; u < KW; ++u) vk[u] = nk[u];
+        const double2 p = np;
+        if (i + 4 < r1) {
+            const size_t en = e + (size_t)4 * Bp;
+#pragma unroll
+            for (int u = 0; u < KW; ++u) {
+                const int k = kg + 4 * u;
+                if (k >= 1 && k < m) nk[u] = V[(size_t)k * nB + en];
+            }
+            if (kg == 0) np = d.psi[en];
+        }
+        double pr = 0.0, pi = 0.0;
+        if (kg == 0) {
+            pr = y[0].x * p.x - y[0].y * p.y;
+            pi = y[0].x * p.y + y[0].y * p.x;
         }
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
