@@ -192,7 +192,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     size_t nq = std::max<size_t>(2 * (s->m + 1) + 1, dq::kMaxGroups + 2);
     size_t part = std::max(nq * d.ntile_e, (size_t)2 * d.ntile_s) * d.Bp;
     ALLOC(&d.part, part);
-    ALLOC(&d.red, nq * d.Bp);
+    ALLOC(&d.red, (nq + 2) * d.Bp);
     ALLOC(&d.step, 1);
     // trajectories: ids 0..B-1, all sites active; padding trajectories inactive
     {

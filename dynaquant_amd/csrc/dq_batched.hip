@@ -73,7 +73,7 @@ template <typename VT>
 __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
 {
     using V2 = typename BVec2<VT>::T;
-    __shared__ VT s_ar[4][kRB][64], s_ai[4][kRB][64];
+    __shared__ VT s_ar[2][kRB][64], s_ai[2][kRB][64];
     const int ntt = d.Bp / 64;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_s * ntt) return;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
     // the four waves take contiguous quarters (multiples of 8) of the union
     const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
     const int len = u1 - u0;
-    const int quarter = ((len + 31) / 32) * 8;
+    const int quarter = ((len + 63) / 64) * 16;
     const int w0 = min(u1, u0 + wave * quarter), w1 = min(u1, w0 + quarter);
     VT ar[kRB], ai[kRB];
 #pragma unroll
@@ -100,25 +100,27 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
     for (int cb = w0; cb < w1; cb += 64) {
         const int lim = min(64, w1 - cb);
         const int mc = lane < lim ? d.rb_col[cb + lane] : 0;
-        for (int q = 0; q < lim; q += 8) {
+        for (int q = 0; q < lim; q += 16) {
+            // 16 gathers and the 16 x kRB values of these columns in flight
             const size_t vb = (size_t)(cb + q) * kRB;
             const size_t vend = (size_t)w1 * kRB;
-            const VT mv0 = vb + lane < vend ? rv[vb + lane] : (VT)0;
-            const VT mv1 = vb + 64 + lane < vend ? rv[vb + 64 + lane] : (VT)0;
-            V2 xv[8];
+            VT mv[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int h = 0; h < 4; ++h) mv[h] = vb + 64 * h + lane < vend ? rv[vb + 64 * h + lane] : (VT)0;
+            V2 xv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
                 if (q + u < lim) {
                     const int c = __builtin_amdgcn_readlane(mc, q + u);
                     xv[u] = x[(size_t)c * Bp + b];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 16; ++u) {
                 if (q + u < lim) {
 #pragma unroll
                     for (int r = 0; r < kRB; ++r) {
-                        const VT v = u < 4 ? bcast(mv0, u * kRB + r) : bcast(mv1, (u - 4) * kRB + r);
+                        const VT v = bcast(mv[u >> 2], (u & 3) * kRB + r);
                         ar[r] += v * xv[u].x;
                         ai[r] += v * xv[u].y;
                     }
@@ -126,8 +128,26 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
             }
         }
     }
+    // deterministic tree over the waves: (w0 + w2) + (w1 + w3)
+    if (wave >= 2) {
 #pragma unroll
-    for (int r = 0; r < kRB; ++r) { s_ar[wave][r][lane] = ar[r]; s_ai[wave][r][lane] = ai[r]; }
+        for (int r = 0; r < kRB; ++r) { s_ar[wave - 2][r][lane] = ar[r]; s_ai[wave - 2][r][lane] = ai[r]; }
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { ar[r] += s_ar[wave][r][lane]; ai[r] += s_ai[wave][r][lane]; }
+    }
+    __syncthreads();
+    if (wave == 1) {
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { s_ar[0][r][lane] = ar[r]; s_ai[0][r][lane] = ai[r]; }
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { s_ar[1][r][lane] = ar[r] + s_ar[0][r][lane]; s_ai[1][r][lane] = ai[r] + s_ai[0][r][lane]; }
+    }
     __syncthreads();
     // epilogue: wave w finishes rows w, w+4, ... of the block
     const int nact = d.nact[b];
@@ -138,8 +158,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
         const int r = wave + 4 * rr;
         const int i = rb * kRB + r;
         if (i < d.n) {
-            const VT sr = (s_ar[0][r][lane] + s_ar[1][r][lane]) + (s_ar[2][r][lane] + s_ar[3][r][lane]);
-            const VT si = (s_ai[0][r][lane] + s_ai[1][r][lane]) + (s_ai[2][r][lane] + s_ai[3][r][lane]);
+            const VT sr = s_ar[1][r][lane];
+            const VT si = s_ai[1][r][lane];
             const size_t e = (size_t)i * Bp + b;
             const V2 xi = x[e];
             const VT di = diag[e];
@@ -181,33 +201,12 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
 {
     __shared__ double s_sj[64];
     __shared__ double2 s_G[kMaxM + 1][64];
-    __shared__ double lds[4][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
-    // four independent chains per lane keep the tile loads in flight
-    double c0v[4] = {0.0, 0.0, 0.0, 0.0}, c1v[4] = {0.0, 0.0, 0.0, 0.0};
-    int t = wave;
-    for (; t + 12 < d.ntile_s; t += 16) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            c0v[u] += d.part[(size_t)(t + 4 * u) * Bp + b];
-            c1v[u] += d.part[((size_t)d.ntile_s + t + 4 * u) * Bp + b];
-        }
-    }
-    for (; t < d.ntile_s; t += 4) {
-        c0v[0] += d.part[(size_t)t * Bp + b];
-        c1v[0] += d.part[((size_t)d.ntile_s + t) * Bp + b];
-    }
-    const double c0 = (c0v[0] + c0v[1]) + (c0v[2] + c0v[3]);
-    const double c1 = (c1v[0] + c1v[1]) + (c1v[2] + c1v[3]);
-    lds[wave][lane] = c0;
-    __syncthreads();
-    const double cr = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
-    __syncthreads();
-    lds[wave][lane] = c1;
-    __syncthreads();
-    const double ci = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
+    // <v_j|w> reduced over the SpMM tiles by k_reduce_wide into red[kRedC..]
+    const double cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
+    const double ci = d.red[((size_t)kRedC(d.m) + 1) * Bp + b];
     const bool writer = blockIdx.y == 0;
     if (wave == 0) {
         double sj = 1.0;
@@ -264,8 +263,41 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     }
 }
 
+// red[dst + q][b] = sum over tiles of part[q][tile][b]: 16 waves per block,
+// four load chains per lane, fixed combination order
+__global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
+{
+    __shared__ double lds[16][64];
+    const int q = blockIdx.y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.x * 64 + lane;
+    const int Bp = d.Bp;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = wave;
+    for (; t + 48 < ntile; t += 64) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] += d.part[((size_t)q * ntile + t + 16 * u) * Bp + b];
+    }
+    for (; t < ntile; t += 16) a[0] += d.part[((size_t)q * ntile + t) * Bp + b];
+    lds[wave][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (wave == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += lds[w][lane];
+        d.red[((size_t)dst + q) * Bp + b] = s;
+    }
+}
+
+hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_wide, dim3(d.Bp / 64, nq), dim3(1024), 0, s, d, ntile, dst);
+    return hipGetLastError();
+}
+
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 {
+    hipError_t e = launch_reduce_wide(d, 2, d.ntile_s, kRedC(d.m), s);
+    if (e != hipSuccess) return e;
     const int ny = j > 0 ? (j + 3) / 4 : 1;
     hipLaunchKernelGGL(k_scalars_hb, dim3(d.Bp / 64, ny), dim3(kThreads), 0, s, d, j);
     return hipGetLastError();

@@ -78,6 +78,9 @@ struct Dev {
 
 enum StatusBits { kStatusScaling = 1, kStatusNonfinite = 2 };
 
+// first row of red[] holding the reduced <v_j|w> (after the Gram sums)
+__host__ __device__ inline int kRedC(int m) { return 2 * (m + 1) + 1; }
+
 // launch wrappers (dq_kernels.hip)
 hipError_t launch_init_disorder(const Dev& d, hipStream_t s);
 hipError_t launch_prologue(const Dev& d, int renorm, int noise, int record, int copy_v0, hipStream_t s);
@@ -88,6 +91,7 @@ hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s);
 // batched variants (dq_batched.hip)
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s);
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s);
+hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
 hipError_t launch_combine_b(const Dev& d, hipStream_t s);
 hipError_t launch_pade(const Dev& d, hipStream_t s);
