@@ -153,7 +153,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.mixed = cfg->precision == DQ_PREC_MIXED;
     d.plain = plain;
     d.rt_e = plain ? 1024 : 128;
-    d.rt_s = plain ? 64 : dq::kRB;       // batched: one row block per workgroup
+    d.rt_s = plain ? 64 : 4 * dq::kRB;   // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.S = cfg->ou_substeps;

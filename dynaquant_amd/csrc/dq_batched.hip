@@ -1,7 +1,8 @@
 // dq_batched.hip -- batched (B >= 64 padded) kernels of the Krylov cycle.
 //
-// SpMM: one wave per block of 16 rows x 64 trajectories over the union of
-// the rows' columns (register reuse of every gathered x run).
+// SpMM: one wave per block of kRB rows x 64-128 trajectories over the union
+// of the rows' columns (register reuse of every gathered x run), with the
+// sparse structure read through wave-uniform scalar loads.
 //
 // Update and combine: a wave covers 16 trajectories x 4 groups of Krylov
 // vectors (lane = 16 * kgroup + t).  Each lane streams only the vectors of
@@ -62,132 +63,157 @@ __device__ __forceinline__ void bstore_partials(const double (&acc)[NQ], int nq,
 
 // ---------------------------------------------------------------------------
 // SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory).
-// Row-blocked: a wave owns kRB consecutive rows x 64 trajectories and walks
-// the union of their columns, so each gathered x run (64 trajectories, one
-// contiguous 512 B / 1 KB segment) feeds kRB rows from registers.  For the
-// dipole tubes kRB = 16 cuts the gathers per row from ~61 to ~10.  Column
-// indices and the kRB values per column are fetched lane-parallel and
-// broadcast with readlane.
+// Row-blocked: a wave owns kRB consecutive rows and walks the union of their
+// columns, so each gathered x run (a contiguous 512 B / 1 KB segment of
+// trajectories) feeds kRB rows from registers.  For the dipole tubes kRB = 4
+// cuts the gathers per row from ~61 to ~21 at 1.36x the FMAs of plain CSR.
 // ---------------------------------------------------------------------------
-template <typename VT>
+template <typename VT> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+
+// Each wave owns one row block (kRB rows) x 64*TPL trajectories; the four
+// waves of a workgroup take consecutive row blocks.  Column offsets and the
+// kRB values of each union column are wave-uniform scalar loads, so a gather
+// costs one vector memory instruction addressed as uniform base + lane offset
+// and the vector ALU only runs the FMAs.  TPL trajectories per lane: 2 in FP32
+// (one 16-byte load per lane), 1 in FP64.
+template <typename VT, int TPL>
 __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
 {
     using V2 = typename BVec2<VT>::T;
-    __shared__ VT s_ar[2][kRB][64], s_ai[2][kRB][64];
-    const int ntt = d.Bp / 64;
+    constexpr int TW = 64 * TPL;                         // trajectories per wave
+    const int ntt = d.Bp / TW;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_s * ntt) return;
-    const int rb = logical / ntt, ttile = logical % ntt;     // one row block per block
+    const int rtile = logical / ntt, ttile = logical % ntt;
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const int b = ttile * 64 + lane;
     const int Bp = d.Bp;
+    const int b0 = ttile * TW + lane * TPL;              // first trajectory of this lane
     const size_t nB = (size_t)d.n * Bp;
-    const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB;
+    const V2* __restrict__ xs = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB + (size_t)ttile * TW;
     V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
+    const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB;
     const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
     const VT* __restrict__ rv;
     if constexpr (sizeof(VT) == 4) rv = d.rb_val32;
     else rv = d.rb_val64;
-    // the four waves take contiguous quarters (multiples of 8) of the union
-    const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
-    const int len = u1 - u0;
-    const int quarter = ((len + 63) / 64) * 16;
-    const int w0 = min(u1, u0 + wave * quarter), w1 = min(u1, w0 + quarter);
-    VT ar[kRB], ai[kRB];
+    const int32_t* __restrict__ rc = d.rb_col;
+    const int nblk = (d.n + kRB - 1) / kRB;
+    const int rb = rtile * 4 + wave;
+    double acc[TPL][2];
 #pragma unroll
-    for (int r = 0; r < kRB; ++r) { ar[r] = 0; ai[r] = 0; }
-    for (int cb = w0; cb < w1; cb += 64) {
-        const int lim = min(64, w1 - cb);
-        const int mc = lane < lim ? d.rb_col[cb + lane] : 0;
-        for (int q = 0; q < lim; q += 16) {
-            // 16 gathers and the 16 x kRB values of these columns in flight
-            const size_t vb = (size_t)(cb + q) * kRB;
-            const size_t vend = (size_t)w1 * kRB;
-            VT mv[4];
+    for (int t = 0; t < TPL; ++t) { acc[t][0] = 0.0; acc[t][1] = 0.0; }
+    if (rb < nblk) {
+        const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+        VT ar[TPL][kRB], ai[TPL][kRB];
 #pragma unroll
-            for (int h = 0; h < 4; ++h) mv[h] = vb + 64 * h + lane < vend ? rv[vb + 64 * h + lane] : (VT)0;
-            V2 xv[16];
+        for (int t = 0; t < TPL; ++t)
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                if (q + u < lim) {
-                    const int c = __builtin_amdgcn_readlane(mc, q + u);
-                    xv[u] = x[(size_t)c * Bp + b];
+            for (int r = 0; r < kRB; ++r) { ar[t][r] = 0; ai[t][r] = 0; }
+        int p = u0;
+        for (; p + 8 <= u1; p += 8) {
+            VT xr[8][TPL], xi[8][TPL];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const V2* xc = xs + (size_t)uniform(rc[p + u]) * Bp;
+                if constexpr (TPL == 2) {
+                    const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
+                    xr[u][0] = q.x; xi[u][0] = q.y; xr[u][1] = q.z; xi[u][1] = q.w;
+                } else {
+                    const V2 q = xc[lane];
+                    xr[u][0] = q.x; xi[u][0] = q.y;
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                if (q + u < lim) {
+            for (int u = 0; u < 8; ++u) {
 #pragma unroll
-                    for (int r = 0; r < kRB; ++r) {
-                        const VT v = bcast(mv[u >> 2], (u & 3) * kRB + r);
-                        ar[r] += v * xv[u].x;
-                        ai[r] += v * xv[u].y;
+                for (int r = 0; r < kRB; ++r) {
+                    const VT v = rv[(size_t)(p + u) * kRB + r];
+#pragma unroll
+                    for (int t = 0; t < TPL; ++t) {
+                        ar[t][r] += v * xr[u][t];
+                        ai[t][r] += v * xi[u][t];
+                    }
+                }
+            }
+        }
+        for (; p < u1; ++p) {
+            const V2* xc = xs + (size_t)uniform(rc[p]) * Bp;
+            VT xr[TPL], xi[TPL];
+            if constexpr (TPL == 2) {
+                const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
+                xr[0] = q.x; xi[0] = q.y; xr[1] = q.z; xi[1] = q.w;
+            } else {
+                const V2 q = xc[lane];
+                xr[0] = q.x; xi[0] = q.y;
+            }
+#pragma unroll
+            for (int r = 0; r < kRB; ++r) {
+                const VT v = rv[(size_t)p * kRB + r];
+#pragma unroll
+                for (int t = 0; t < TPL; ++t) {
+                    ar[t][r] += v * xr[t];
+                    ai[t][r] += v * xi[t];
+                }
+            }
+        }
+        const VT g = (VT)d.g;
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) {
+            const int i = rb * kRB + r;
+            if (i < d.n) {
+#pragma unroll
+                for (int t = 0; t < TPL; ++t) {
+                    const int b = b0 + t;
+                    const size_t e = (size_t)i * Bp + b;
+                    const V2 xv = x[e];
+                    const VT di = diag[e];
+                    VT wr = di * xv.x + g * ar[t][r];
+                    VT wi = di * xv.y + g * ai[t][r];
+                    if (i >= d.nact[b]) { wr = 0; wi = 0; }
+                    if (!last) {
+                        V2 w;
+                        w.x = wr;
+                        w.y = wi;
+                        y[e] = w;
+                        const double xr = (double)xv.x, xm = (double)xv.y, dr = (double)wr, dm = (double)wi;
+                        acc[t][0] += xr * dr + xm * dm;
+                        acc[t][1] += xr * dm - xm * dr;
+                    } else {
+                        acc[t][0] += (double)wr * (double)wr + (double)wi * (double)wi;
                     }
                 }
             }
         }
     }
-    // deterministic tree over the waves: (w0 + w2) + (w1 + w3)
-    if (wave >= 2) {
+    // partial sums of the workgroup (4 waves = 4 row blocks), per trajectory
+    __shared__ double lds[4][2][TPL][64];
 #pragma unroll
-        for (int r = 0; r < kRB; ++r) { s_ar[wave - 2][r][lane] = ar[r]; s_ai[wave - 2][r][lane] = ai[r]; }
-    }
-    __syncthreads();
-    if (wave < 2) {
-#pragma unroll
-        for (int r = 0; r < kRB; ++r) { ar[r] += s_ar[wave][r][lane]; ai[r] += s_ai[wave][r][lane]; }
-    }
-    __syncthreads();
-    if (wave == 1) {
-#pragma unroll
-        for (int r = 0; r < kRB; ++r) { s_ar[0][r][lane] = ar[r]; s_ai[0][r][lane] = ai[r]; }
-    }
+    for (int t = 0; t < TPL; ++t) { lds[wave][0][t][lane] = acc[t][0]; lds[wave][1][t][lane] = acc[t][1]; }
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-        for (int r = 0; r < kRB; ++r) { s_ar[1][r][lane] = ar[r] + s_ar[0][r][lane]; s_ai[1][r][lane] = ai[r] + s_ai[0][r][lane]; }
-    }
-    __syncthreads();
-    // epilogue: wave w finishes rows w, w+4, ... of the block
-    const int nact = d.nact[b];
-    const VT g = (VT)d.g;
-    double acc0 = 0.0, acc1 = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            if (q == 1 && last) break;
 #pragma unroll
-    for (int rr = 0; rr < kRB / 4; ++rr) {
-        const int r = wave + 4 * rr;
-        const int i = rb * kRB + r;
-        if (i < d.n) {
-            const VT sr = s_ar[1][r][lane];
-            const VT si = s_ai[1][r][lane];
-            const size_t e = (size_t)i * Bp + b;
-            const V2 xi = x[e];
-            const VT di = diag[e];
-            VT wr = di * xi.x + g * sr;
-            VT wi = di * xi.y + g * si;
-            if (i >= nact) { wr = 0; wi = 0; }
-            if (!last) {
-                V2 w;
-                w.x = wr;
-                w.y = wi;
-                y[e] = w;
-                const double xr = (double)xi.x, xm = (double)xi.y, dr = (double)wr, dm = (double)wi;
-                acc0 += xr * dr + xm * dm;
-                acc1 += xr * dm - xm * dr;
-            } else {
-                acc0 += (double)wr * (double)wr + (double)wi * (double)wi;
+            for (int t = 0; t < TPL; ++t) {
+                const double v = (lds[0][q][t][lane] + lds[1][q][t][lane]) + (lds[2][q][t][lane] + lds[3][q][t][lane]);
+                d.part[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
             }
         }
     }
-    double out[2] = {acc0, acc1};
-    bstore_partials<2>(out, last ? 1 : 2, d.part, d.ntile_s, rb, Bp, b);
 }
 
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
-    dim3 grid(bpad8(d.ntile_s * (d.Bp / 64)));
-    if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float>), grid, dim3(kThreads), 0, s, d, j, last);
-    else hipLaunchKernelGGL((k_spmm_b<double>), grid, dim3(kThreads), 0, s, d, j, last);
+    if (d.mixed && d.Bp % 128 == 0) {
+        hipLaunchKernelGGL((k_spmm_b<float, 2>), dim3(bpad8(d.ntile_s * (d.Bp / 128))), dim3(kThreads), 0, s, d, j, last);
+    } else if (d.mixed) {
+        hipLaunchKernelGGL((k_spmm_b<float, 1>), dim3(bpad8(d.ntile_s * (d.Bp / 64))), dim3(kThreads), 0, s, d, j, last);
+    } else {
+        hipLaunchKernelGGL((k_spmm_b<double, 1>), dim3(bpad8(d.ntile_s * (d.Bp / 64))), dim3(kThreads), 0, s, d, j, last);
+    }
     return hipGetLastError();
 }
 

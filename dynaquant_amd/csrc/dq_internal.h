@@ -14,7 +14,7 @@ namespace dq {
 constexpr int kMaxM = 32;          // Krylov depth limit (Pade matrix <= 34)
 constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
-constexpr int kRB = 16;            // rows per block of the row-blocked SpMM
+constexpr int kRB = 4;             // rows per block of the row-blocked SpMM
 
 struct Dev {
     int n, Bp, B, m;
