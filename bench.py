@@ -71,6 +71,48 @@ def update_bytes(n, b, m, mixed):
     return sum((j + 3) * n * b * bv for j in range(m))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_N12000_B256_m24.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` ("update" or "spmm") from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
+    factor recorded there); None when that summary is absent"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                key, _, val = line.partition("=")
+                if key.strip() == "traffic_bytes_per_launch." + kernel:
+                    return float(val)
+    except OSError:
+        pass
+    return None
+
+
+def traffic_probe(dq, t, steps=2):
+    """workload for the rocprofv3 --pmc runs: a few batched steps without the
+    graph, then one 256 MiB device copy whose counters give the scale factor"""
+    import ctypes
+    s = make_system(dq, t, B, dq.MIXED, steps + 1)
+    s.run(steps, graph=False)
+    s.close()
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    ok = hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    ok = ok and hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # device to device
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("traffic probe: HIP copy failed")
+    print("# traffic probe: %d batched steps, then one device copy of %d bytes" % (steps, nbytes),
+          file=sys.stderr, flush=True)
+
+
 def cpu_baseline(t, seconds_target=15.0):
     """FP64 CPU oracle on the host cores: one trajectory per OpenMP thread"""
     from oracle import pyoracle as O
@@ -83,7 +125,7 @@ def cpu_baseline(t, seconds_target=15.0):
                 t["groups"], 8, nthreads=cores)
     dt = time.perf_counter() - t0
     steps2 = max(2, int(steps * seconds_target / max(dt, 1e-3)))
-    steps2 = min(steps2, 200)
+    steps2 = min(steps2, 2000)
     t0 = time.perf_counter()
     O.run_batch(p, sites, t["rp"], t["col"], t["val"], t["psi0"], np.arange(cores), steps2, EVERY,
                 t["groups"], 8, nthreads=cores)
@@ -102,6 +144,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only the short workload profiled for HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; trajectories shard across GPUs with no "
@@ -113,6 +157,9 @@ def main():
     n, nnz = t["n"], len(t["col"])
     K, W = args.steps, args.warmup
     print("# workload N=%d nnz_off=%d B=%d m=%d dt=%g" % (n, nnz, B, M, DT), file=sys.stderr, flush=True)
+    if args.traffic_probe:
+        traffic_probe(dq, t)
+        return
 
     # ---- headline: batched mixed precision, hipGraph per step
     s = make_system(dq, t, B, dq.MIXED, W + K + args.profile_steps)
@@ -140,16 +187,15 @@ def main():
     dominant = max(totals, key=totals.get)
     print("# kernel ms per step: " + ", ".join("%s %.3f" % (k, v / cyc) for k, v in totals.items()),
           file=sys.stderr, flush=True)
-    if dominant == "spmm":
-        roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_spmm<float>",
-                "bytes_per_launch": b_spmm, "avg_launch_ms": spmm_ms}
-    else:
-        roof = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": upd_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_update<false,float,24>",
+    upd_roof = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("update"),
+                "kernel": "dq::k_update_b<float,KW> (KW = 2/4/6 for j+1 <= 8/16/24)",
                 "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms}
-    spmm_roof = {"achieved": spmm_gbs, "frac": spmm_gbs / HBM_PEAK_GBS, "bytes_per_launch": b_spmm,
+    spmm_roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("spmm"),
+                 "kernel": "dq::k_spmm_b<float,2>", "bytes_per_launch": b_spmm,
                  "avg_launch_ms": spmm_ms}
+    roof = spmm_roof if dominant == "spmm" else upd_roof
 
     # ---- plain FP64 single-trajectory path, same tube
     p = make_system(dq, t, 1, dq.FP64, 20 + args.plain_steps)

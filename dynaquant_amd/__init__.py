@@ -26,6 +26,7 @@ EXPORTS = [
     "dq_propagate_step", "dq_run", "dq_get_step", "dq_get_populations", "dq_get_corr",
     "dq_get_error_sum", "dq_get_diagonal", "dq_spmv", "dq_pade_tbar", "dq_spectrum", "dq_fft",
     "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times", "dq_adjoint",
+    "dq_get_tuning", "dq_set_tuning", "dq_autotune",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -46,6 +47,16 @@ class Config(C.Structure):
         ("dt_fs", C.c_double), ("hbar_cm_fs", C.c_double), ("coupling_scale", C.c_double),
         ("sigma_static", C.c_double), ("breakdown_tol", C.c_double), ("seed", C.c_uint64),
     ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("spmm_rows_per_wave", C.c_int32), ("spmm_traj_per_lane", C.c_int32),
+        ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("reserved", C.c_int32 * 4),
+    ]
+
+
+TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows"]
 
 
 class TubeSpec(C.Structure):
@@ -215,6 +226,28 @@ class System:
 
     def set_profiling(self, on=True):
         _check(lib().dq_set_profiling(self._h, int(on)))
+
+    def tuning(self):
+        """current launch choices of the batched kernels"""
+        t = Tuning()
+        _check(lib().dq_get_tuning(self._h, C.byref(t)))
+        return {k: int(getattr(t, k)) for k in TUNING_KEYS}
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        _check(lib().dq_get_tuning(self._h, C.byref(t)))
+        for k, v in kw.items():
+            if k not in TUNING_KEYS:
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        _check(lib().dq_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, reps=3, cache=None):
+        """time every launch choice on this system, apply the fastest, return it"""
+        t = Tuning()
+        path = None if cache is None else os.fsencode(cache)
+        _check(lib().dq_autotune(self._h, int(reps), path, C.byref(t)))
+        return {k: int(getattr(t, k)) for k in TUNING_KEYS}
 
     def kernel_times(self):
         """{class: (total ms, launches)} accumulated in profiling mode"""

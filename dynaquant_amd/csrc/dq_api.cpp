@@ -119,9 +119,68 @@ struct dq_system {
     }
 };
 
-extern "C" {
+extern "C" const char* dq_last_error(void) { return g_err.c_str(); }
 
-const char* dq_last_error(void) { return g_err.c_str(); }
+// Row-blocked copy of the couplings for the batched SpMM: per block of R rows
+// the sorted union of their columns and R values per union column (0 where a
+// row lacks that column), from the host CSR kept by set_hamiltonian.
+static dq_status build_row_blocks(dq_system* sys, int R)
+{
+    Dev& d = sys->d;
+    const int n = sys->n, nblk = (n + R - 1) / R;
+    const int32_t* rowptr = sys->rowptr.data();
+    const int32_t* col = sys->col.data();
+    const double* val = sys->val.data();
+    std::vector<int32_t> rbp(nblk + 1, 0), rbc;
+    std::vector<double> rbv64;
+    std::vector<int32_t> u;
+    for (int k = 0; k < nblk; ++k) {
+        const int r0 = k * R, r1 = std::min(n, r0 + R);
+        u.clear();
+        for (int i = r0; i < r1; ++i) u.insert(u.end(), col + rowptr[i], col + rowptr[i + 1]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        const size_t base = rbc.size();
+        rbc.insert(rbc.end(), u.begin(), u.end());
+        rbv64.resize(rbc.size() * R, 0.0);
+        for (int i = r0; i < r1; ++i)
+            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
+                rbv64[(base + pos) * R + (i - r0)] = val[p];
+            }
+        rbp[k + 1] = (int32_t)rbc.size();
+    }
+    std::vector<float> rbv32(rbv64.size());
+    for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->release(*(int32_t**)&d.rb_ptr);
+    sys->release(*(int32_t**)&d.rb_col);
+    sys->release(*(float**)&d.rb_val32);
+    sys->release(*(double**)&d.rb_val64);
+    int32_t *drbp, *drbc;
+    float* drv32;
+    double* drv64;
+    DQ_TRY(sys->alloc(&drbp, rbp.size()));
+    DQ_TRY(sys->alloc(&drbc, std::max<size_t>(rbc.size(), 1)));
+    DQ_TRY(sys->alloc(&drv32, std::max<size_t>(rbv32.size(), 1)));
+    DQ_TRY(sys->alloc(&drv64, std::max<size_t>(rbv64.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(drbp, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
+    if (!rbc.empty()) {
+        HIP_TRY(hipMemcpyAsync(drbc, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(drv32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(drv64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.rb_ptr = drbp; d.rb_col = drbc; d.rb_val32 = drv32; d.rb_val64 = drv64;
+    d.rb = R;
+    if (!d.plain) {
+        d.rt_s = 4 * R;
+        d.ntile_s = (n + d.rt_s - 1) / d.rt_s;
+    }
+    return DQ_OK;
+}
+
+extern "C" {
 
 dq_status dq_system_create(const dq_config* cfg, dq_system** out)
 {
@@ -153,7 +212,10 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.mixed = cfg->precision == DQ_PREC_MIXED;
     d.plain = plain;
     d.rt_e = plain ? 1024 : 128;
-    d.rt_s = plain ? 64 : 4 * dq::kRB;   // batched: one row block per wave
+    d.rb = 4;
+    d.spmm_tpl = (d.mixed && d.Bp % 128 == 0) ? 2 : 1;
+    d.spmm_phases = 1;
+    d.rt_s = plain ? 64 : 4 * d.rb;      // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.S = cfg->ou_substeps;
@@ -190,7 +252,10 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     ALLOC(&d.broke, (size_t)d.Bp);
     ALLOC(&d.status, 1);
     size_t nq = std::max<size_t>(2 * (s->m + 1) + 1, dq::kMaxGroups + 2);
-    size_t part = std::max(nq * d.ntile_e, (size_t)2 * d.ntile_s) * d.Bp;
+    // partial sums sized for the smallest tiles the tuning allows
+    const size_t te_max = plain ? (size_t)d.ntile_e : (size_t)((d.n + dq::kRowsMin - 1) / dq::kRowsMin);
+    const size_t ts_max = plain ? (size_t)d.ntile_s : (size_t)((d.n + 4 * dq::kRBMin - 1) / (4 * dq::kRBMin));
+    size_t part = std::max(nq * te_max, (size_t)2 * ts_max) * d.Bp;
     ALLOC(&d.part, part);
     ALLOC(&d.red, (nq + 2) * d.Bp);
     ALLOC(&d.step, 1);
@@ -280,48 +345,7 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
         HIP_TRY(hipMemcpyAsync(v32, f32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, sys->stream));
     }
     HIP_TRY(hipMemcpyAsync(es, esv.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
-    // row-blocked copy: per block of kRB rows the sorted union of columns and
-    // kRB values per union column
-    const int R = dq::kRB, nblk = (n + R - 1) / R;
-    std::vector<int32_t> rbp(nblk + 1, 0), rbc;
-    std::vector<double> rbv64;
-    for (int k = 0; k < nblk; ++k) {
-        const int r0 = k * R, r1 = std::min(n, r0 + R);
-        std::vector<int32_t> u;
-        for (int i = r0; i < r1; ++i) u.insert(u.end(), col + rowptr[i], col + rowptr[i + 1]);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        const size_t base = rbc.size();
-        rbc.insert(rbc.end(), u.begin(), u.end());
-        rbv64.resize(rbc.size() * R, 0.0);
-        for (int i = r0; i < r1; ++i)
-            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-                const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
-                rbv64[(base + pos) * R + (i - r0)] = val[p];
-            }
-        rbp[k + 1] = (int32_t)rbc.size();
-    }
-    std::vector<float> rbv32(rbv64.size());
-    for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
-    sys->release(*(int32_t**)&d.rb_ptr);
-    sys->release(*(int32_t**)&d.rb_col);
-    sys->release(*(float**)&d.rb_val32);
-    sys->release(*(double**)&d.rb_val64);
-    int32_t *drbp, *drbc;
-    float* drv32;
-    double* drv64;
-    DQ_TRY(sys->alloc(&drbp, rbp.size()));
-    DQ_TRY(sys->alloc(&drbc, std::max<size_t>(rbc.size(), 1)));
-    DQ_TRY(sys->alloc(&drv32, std::max<size_t>(rbv32.size(), 1)));
-    DQ_TRY(sys->alloc(&drv64, std::max<size_t>(rbv64.size(), 1)));
-    HIP_TRY(hipMemcpyAsync(drbp, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
-    if (!rbc.empty()) {
-        HIP_TRY(hipMemcpyAsync(drbc, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(drv32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(drv64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(sys->stream));
-    d.rb_ptr = drbp; d.rb_col = drbc; d.rb_val32 = drv32; d.rb_val64 = drv64;
+    DQ_TRY(build_row_blocks(sys, d.rb));
     d.rowptr = rp; d.col = cl; d.val64 = v64; d.val32 = v32; d.e_site = es;
     sys->have_h = true;
     return DQ_OK;
@@ -1007,6 +1031,209 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     HIP_TRY(dq::launch_set_unit_norm(d, st));
     cleanup();
 #undef AGET
+    return DQ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch tuning and the autotuner
+// ---------------------------------------------------------------------------
+static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* why)
+{
+    const Dev& d = sys->d;
+    if (t.spmm_rows_per_wave != 2 && t.spmm_rows_per_wave != 4 && t.spmm_rows_per_wave != 8) {
+        *why = "spmm_rows_per_wave must be 2, 4 or 8";
+        return false;
+    }
+    if (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) {
+        *why = "spmm_traj_per_lane must be 1 or 2";
+        return false;
+    }
+    if (t.spmm_traj_per_lane == 2 && !(d.mixed && d.Bp % 128 == 0)) {
+        *why = "spmm_traj_per_lane = 2 needs mixed precision and a padded batch that is a multiple of 128";
+        return false;
+    }
+    const int nslice = d.Bp / (64 * t.spmm_traj_per_lane);
+    if ((t.spmm_phases != 1 && t.spmm_phases != 2 && t.spmm_phases != 4) || nslice % t.spmm_phases) {
+        *why = "spmm_phases must be 1, 2 or 4 and divide the number of trajectory slices";
+        return false;
+    }
+    if (t.tile_rows != 64 && t.tile_rows != 128 && t.tile_rows != 256) {
+        *why = "tile_rows must be 64, 128 or 256";
+        return false;
+    }
+    return true;
+}
+
+static void get_tuning(const dq_system* sys, dq_tuning* t)
+{
+    std::memset(t, 0, sizeof(*t));
+    t->spmm_rows_per_wave = sys->d.rb;
+    t->spmm_traj_per_lane = sys->d.spmm_tpl;
+    t->spmm_phases = sys->d.spmm_phases;
+    t->tile_rows = sys->d.plain ? 0 : sys->d.rt_e;
+}
+
+static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
+{
+    Dev& d = sys->d;
+    if (t.spmm_rows_per_wave != d.rb) {
+        if (sys->have_h) {
+            DQ_TRY(build_row_blocks(sys, t.spmm_rows_per_wave));
+        } else {
+            d.rb = t.spmm_rows_per_wave;
+            d.rt_s = 4 * d.rb;
+            d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
+        }
+    }
+    d.spmm_tpl = t.spmm_traj_per_lane;
+    d.spmm_phases = t.spmm_phases;
+    d.rt_e = t.tile_rows;
+    d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
+    return DQ_OK;
+}
+
+// one Lanczos cycle without the kernels that touch the propagated state
+// (noise, Pade error sum, renormalisation): SpMMs, scalars, updates, reductions
+// and the combine (whose psi output the caller restores)
+static hipError_t enqueue_tuning_cycle(dq_system* sys)
+{
+    const Dev& d = sys->d;
+    hipStream_t st = sys->stream;
+    hipError_t e;
+    for (int j = 0; j < d.m; ++j) {
+        if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
+        if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
+        if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
+        if ((e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;
+    }
+    if ((e = dq::launch_spmm(d, d.m, 1, st)) != hipSuccess) return e;
+    return dq::launch_combine(d, st);
+}
+
+static std::string tuning_key(const dq_system* sys)
+{
+    hipDeviceProp_t prop;
+    int dev = 0;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+        for (char& c : arch) if (c == ' ') c = '_';
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s cu=%d n=%d nnz=%lld bp=%d m=%d prec=%s", arch.c_str(), cus, sys->n,
+                  (long long)sys->col.size(), sys->Bp, sys->m, sys->d.mixed ? "mixed" : "fp64");
+    return buf;
+}
+
+dq_status dq_get_tuning(dq_system* sys, dq_tuning* out)
+{
+    if (!sys || !out) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    get_tuning(sys, out);
+    return DQ_OK;
+}
+
+dq_status dq_set_tuning(dq_system* sys, const dq_tuning* t)
+{
+    if (!sys || !t) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (sys->d.plain) return DQ_OK;
+    std::string why;
+    if (!tuning_valid(sys, *t, &why)) return fail(DQ_ERR_INVALID_ARG, why);
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    return apply_tuning(sys, *t);
+}
+
+dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_tuning* chosen)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    if (!sys->have_h) return fail(DQ_ERR_STATE, "autotune needs set_hamiltonian first");
+    if (reps < 1) return fail(DQ_ERR_INVALID_ARG, "reps must be >= 1");
+    dq_tuning best;
+    get_tuning(sys, &best);
+    if (sys->d.plain) {
+        if (chosen) *chosen = best;
+        return DQ_OK;
+    }
+    const std::string key = tuning_key(sys);
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "r")) {
+            char line[512];
+            bool hit = false;
+            while (!hit && std::fgets(line, sizeof(line), f)) {
+                if (std::strncmp(line, key.c_str(), key.size()) != 0 || line[key.size()] != ' ') continue;
+                dq_tuning t;
+                std::memset(&t, 0, sizeof(t));
+                std::string why;
+                if (std::sscanf(line + key.size(), "%d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
+                                &t.spmm_phases, &t.tile_rows) == 4 && tuning_valid(sys, t, &why)) {
+                    best = t;
+                    hit = true;
+                }
+            }
+            std::fclose(f);
+            if (hit) {
+                DQ_TRY(dq_set_tuning(sys, &best));
+                if (chosen) *chosen = best;
+                return DQ_OK;
+            }
+        }
+    }
+    Dev& d = sys->d;
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    const size_t nB = (size_t)d.n * d.Bp;
+    double2* saved = nullptr;
+    HIP_TRY(hipMalloc(&saved, sizeof(double2) * nB));
+    dq_tuning orig = best;
+    dq_status st = DQ_OK;
+    double best_ms = 1e300;
+    hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    for (int R : {4, 2, 8}) {
+        for (int tpl : {2, 1}) {
+            for (int ph : {1, 2, 4}) {
+                for (int rows : {128, 64, 256}) {
+                    if (e != hipSuccess || st != DQ_OK) break;
+                    dq_tuning t;
+                    std::memset(&t, 0, sizeof(t));
+                    t.spmm_rows_per_wave = R;
+                    t.spmm_traj_per_lane = tpl;
+                    t.spmm_phases = ph;
+                    t.tile_rows = rows;
+                    std::string why;
+                    if (!tuning_valid(sys, t, &why)) continue;
+                    if ((st = apply_tuning(sys, t)) != DQ_OK) break;
+                    if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;       // warm-up
+                    float ms_min = 1e30f;
+                    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+                        float ms = 0.f;
+                        if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
+                        if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;
+                        if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
+                        if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
+                        if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
+                        ms_min = std::min(ms_min, ms);
+                    }
+                    if (e == hipSuccess && ms_min < best_ms) { best_ms = ms_min; best = t; }
+                }
+            }
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(d.psi, saved, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
+    hipFree(saved);
+    if (e != hipSuccess) return fail(DQ_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    if (st != DQ_OK) { apply_tuning(sys, orig); return st; }
+    DQ_TRY(apply_tuning(sys, best));
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
+                         best.spmm_phases, best.tile_rows);
+            std::fclose(f);
+        }
+    }
+    if (chosen) *chosen = best;
     return DQ_OK;
 }
 

@@ -63,13 +63,19 @@ __device__ __forceinline__ void bstore_partials(const double (&acc)[NQ], int nq,
 
 // ---------------------------------------------------------------------------
 // SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory).
-// Row-blocked: a wave owns kRB consecutive rows and walks the union of their
-// columns, so each gathered x run (a contiguous 512 B / 1 KB segment of
-// trajectories) feeds kRB rows from registers.  For the dipole tubes kRB = 4
-// cuts the gathers per row from ~61 to ~21 at 1.36x the FMAs of plain CSR.
+// Row-blocked: a wave owns kRB consecutive rows (2, 4 or 8, chosen by the
+// autotuner) and walks the union of their columns, so each gathered x run (a
+// contiguous 512 B / 1 KB segment of trajectories) feeds kRB rows from
+// registers.  For the N = 12,000 dipole tube kRB = 4 cuts the gathers per row
+// from 62 to 21 at 1.33x the FMAs of plain CSR; kRB = 8 gives 14 at 1.78x.
 // ---------------------------------------------------------------------------
 template <typename VT> struct Vec4;
 template <> struct Vec4<float> { using T = float4; };
+
+// explicit fused multiply-adds, so that every instantiation (trajectories per
+// lane, rows per wave) rounds each trajectory's sums identically
+__device__ __forceinline__ float fmav(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmav(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Each wave owns one row block (kRB rows) x 64*TPL trajectories; the four
 // waves of a workgroup take consecutive row blocks.  Column offsets and the
@@ -77,15 +83,14 @@ template <> struct Vec4<float> { using T = float4; };
 // costs one vector memory instruction addressed as uniform base + lane offset
 // and the vector ALU only runs the FMAs.  TPL trajectories per lane: 2 in FP32
 // (one 16-byte load per lane), 1 in FP64.
-template <typename VT, int TPL>
-__global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
+template <typename VT, int TPL, int kRB>
+__global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int tt0, int ntt)
 {
     using V2 = typename BVec2<VT>::T;
     constexpr int TW = 64 * TPL;                         // trajectories per wave
-    const int ntt = d.Bp / TW;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_s * ntt) return;
-    const int rtile = logical / ntt, ttile = logical % ntt;
+    const int rtile = logical / ntt, ttile = tt0 + logical % ntt;
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int Bp = d.Bp;
@@ -132,8 +137,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
                     const VT v = rv[(size_t)(p + u) * kRB + r];
 #pragma unroll
                     for (int t = 0; t < TPL; ++t) {
-                        ar[t][r] += v * xr[u][t];
-                        ai[t][r] += v * xi[u][t];
+                        ar[t][r] = fmav(v, xr[u][t], ar[t][r]);
+                        ai[t][r] = fmav(v, xi[u][t], ai[t][r]);
                     }
                 }
             }
@@ -153,8 +158,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
                 const VT v = rv[(size_t)p * kRB + r];
 #pragma unroll
                 for (int t = 0; t < TPL; ++t) {
-                    ar[t][r] += v * xr[t];
-                    ai[t][r] += v * xi[t];
+                    ar[t][r] = fmav(v, xr[t], ar[t][r]);
+                    ai[t][r] = fmav(v, xi[t], ai[t][r]);
                 }
             }
         }
@@ -169,8 +174,8 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
                     const size_t e = (size_t)i * Bp + b;
                     const V2 xv = x[e];
                     const VT di = diag[e];
-                    VT wr = di * xv.x + g * ar[t][r];
-                    VT wi = di * xv.y + g * ai[t][r];
+                    VT wr = fmav(g, ar[t][r], di * xv.x);
+                    VT wi = fmav(g, ai[t][r], di * xv.y);
                     if (i >= d.nact[b]) { wr = 0; wi = 0; }
                     if (!last) {
                         V2 w;
@@ -178,10 +183,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
                         w.y = wi;
                         y[e] = w;
                         const double xr = (double)xv.x, xm = (double)xv.y, dr = (double)wr, dm = (double)wi;
-                        acc[t][0] += xr * dr + xm * dm;
-                        acc[t][1] += xr * dm - xm * dr;
+                        acc[t][0] = __builtin_fma(xm, dm, __builtin_fma(xr, dr, acc[t][0]));
+                        acc[t][1] = __builtin_fma(-xm, dr, __builtin_fma(xr, dm, acc[t][1]));
                     } else {
-                        acc[t][0] += (double)wr * (double)wr + (double)wi * (double)wi;
+                        const double dr = (double)wr, dm = (double)wi;
+                        acc[t][0] = __builtin_fma(dm, dm, __builtin_fma(dr, dr, acc[t][0]));
                     }
                 }
             }
@@ -205,14 +211,30 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last)
     }
 }
 
+template <int RB>
+static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, int ntt, hipStream_t s)
+{
+    const dim3 grid(bpad8(d.ntile_s * ntt)), block(kThreads);
+    for (int ph = 0; ph < phases; ++ph) {
+        if (tpl == 2) hipLaunchKernelGGL((k_spmm_b<float, 2, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        else if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float, 1, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        else hipLaunchKernelGGL((k_spmm_b<double, 1, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+    }
+}
+
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
-    if (d.mixed && d.Bp % 128 == 0) {
-        hipLaunchKernelGGL((k_spmm_b<float, 2>), dim3(bpad8(d.ntile_s * (d.Bp / 128))), dim3(kThreads), 0, s, d, j, last);
-    } else if (d.mixed) {
-        hipLaunchKernelGGL((k_spmm_b<float, 1>), dim3(bpad8(d.ntile_s * (d.Bp / 64))), dim3(kThreads), 0, s, d, j, last);
-    } else {
-        hipLaunchKernelGGL((k_spmm_b<double, 1>), dim3(bpad8(d.ntile_s * (d.Bp / 64))), dim3(kThreads), 0, s, d, j, last);
+    // trajectory slices of 64 * TPL; the slices run as `phases` launches of
+    // ntt slices each, so that one launch's gathered rows fit the L2 of an XCD
+    const int tpl = (d.mixed && d.spmm_tpl == 2 && d.Bp % 128 == 0) ? 2 : 1;
+    const int nslice = d.Bp / (64 * tpl);
+    int phases = d.spmm_phases < 1 ? 1 : d.spmm_phases;
+    while (nslice % phases) --phases;
+    const int ntt = nslice / phases;
+    switch (d.rb) {
+    case 2: launch_spmm_rb<2>(d, j, last, tpl, phases, ntt, s); break;
+    case 8: launch_spmm_rb<8>(d, j, last, tpl, phases, ntt, s); break;
+    default: launch_spmm_rb<4>(d, j, last, tpl, phases, ntt, s); break;
     }
     return hipGetLastError();
 }

@@ -14,7 +14,8 @@ namespace dq {
 constexpr int kMaxM = 32;          // Krylov depth limit (Pade matrix <= 34)
 constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
-constexpr int kRB = 4;             // rows per block of the row-blocked SpMM
+constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
+constexpr int kRowsMin = 64;       // smallest elementwise tile (rows per workgroup)
 
 struct Dev {
     int n, Bp, B, m;
@@ -22,6 +23,9 @@ struct Dev {
     int plain;                      // 1: B == 1 single-trajectory kernels
     int rt_e, ntile_e;              // elementwise kernels: rows per tile, tiles
     int rt_s, ntile_s;              // SpMM kernels
+    int rb;                         // batched SpMM: rows per wavefront (row block)
+    int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
+    int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
     int S;                          // OU substeps
     uint64_t seed;
     double subdt, sqrt_subdt;
@@ -33,12 +37,12 @@ struct Dev {
     const int32_t* col;
     const double* val64;
     const float* val32;
-    // row-blocked copy for the batched SpMM: blocks of kRB rows, the union
-    // of their columns and kRB values per union column (0 where absent)
-    const int32_t* rb_ptr;          // [n/kRB + 1] offsets into rb_col
+    // row-blocked copy for the batched SpMM: blocks of rb rows, the union
+    // of their columns and rb values per union column (0 where absent)
+    const int32_t* rb_ptr;          // [n/rb + 1] offsets into rb_col
     const int32_t* rb_col;          // [total union columns]
-    const float* rb_val32;          // [total][kRB]
-    const double* rb_val64;         // [total][kRB]
+    const float* rb_val32;          // [total][rb]
+    const double* rb_val64;         // [total][rb]
     const double* inv_tau;          // [n]
     const double* coeff;            // [n] sigma*sqrt(2/tau)
     const double* ou_sigma;         // [n]

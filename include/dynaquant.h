@@ -207,6 +207,30 @@ enum {
 dq_status dq_set_profiling(dq_system *sys, int32_t on);
 dq_status dq_get_kernel_times(dq_system *sys, double *ms, int64_t *launches);
 
+/* Launch choices of the batched kernels.  No counterpart in the reference
+ * (its csrmv/zgemv launch shapes are fixed inside cuSPARSE/cuBLAS); these are
+ * the search space of dq_autotune.  Every choice gives the same per-trajectory
+ * arithmetic; tile_rows only regroups the FP64 partial sums.  The plain
+ * single-trajectory path has fixed shapes and ignores them. */
+typedef struct dq_tuning {
+    int32_t spmm_rows_per_wave;  /* 2, 4 or 8 rows whose column union one wavefront gathers */
+    int32_t spmm_traj_per_lane;  /* 1 or 2 (2: FP32 only, padded batch a multiple of 128) */
+    int32_t spmm_phases;         /* trajectory slices of one SpMM run as 1, 2 or 4 launches */
+    int32_t tile_rows;           /* 64, 128 or 256 rows per workgroup of the update, combine
+                                    and noise kernels */
+    int32_t reserved[4];
+} dq_tuning;
+
+dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);
+/* Validates every field; drops a captured graph. */
+dq_status dq_set_tuning(dq_system *sys, const dq_tuning *t);
+/* Times every valid dq_tuning on the system's current Hamiltonian and batch
+ * (reps Lanczos cycles each, the state propagated by dq_run is left as it
+ * was), applies the fastest and returns it in *chosen.  With cache_path, a
+ * line keyed by device, n, nnz, padded batch, m and precision is looked up
+ * first and, after a search, appended.  Needs set_hamiltonian. */
+dq_status dq_autotune(dq_system *sys, int32_t reps, const char *cache_path, dq_tuning *chosen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,111 @@
+"""Launch tuning of the batched kernels: every choice gives the same answer,
+and the autotuner leaves the propagated state alone."""
+import numpy as np
+import pytest
+
+from tests.helpers import make_engine, make_oracle, ring_tube
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(engine, sysd, precision, B, tuning, steps=40):
+    n = sysd["n"]
+    psi0 = np.zeros(n, complex); psi0[0] = 1
+    s = make_engine(engine, sysd, 16, 1.0, B=B, force_batched=True, precision=precision)
+    if tuning:
+        s.set_tuning(**tuning)
+    s.set_state(psi0)
+    grp = (np.arange(n) // 150).astype(np.int32)
+    s.set_observables(grp, 4, 5, steps)
+    s.run(steps, graph=True)
+    out = (s.populations(), s.get_state(), s.tuning())
+    s.close()
+    return out
+
+
+SPMM_CHOICES = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=p)
+                for r in (2, 4, 8) for t in (1, 2) for p in (1, 2)]
+
+
+def test_spmm_choices_mixed(engine):
+    """trajectories per lane and phases leave every per-trajectory sum and every
+    partial-sum tile unchanged: bitwise identical results for one rows-per-wave
+    choice.  Rows per wave regroups the FP64 <v|w> tiles, so across it only the
+    FP32 rounding noise differs"""
+    sysd = ring_tube(engine)
+    ref = {}
+    for c in SPMM_CHOICES:
+        pops, psi, tun = _run(engine, sysd, engine.MIXED, 256, c)
+        for k, v in c.items():
+            assert tun[k] == v
+        r = c["spmm_rows_per_wave"]
+        if r not in ref:
+            ref[r] = (pops, psi)
+            continue
+        print(c, np.abs(pops - ref[r][0]).max())
+        assert np.array_equal(pops, ref[r][0]), c
+        assert np.array_equal(psi, ref[r][1]), c
+    for r in (2, 8):
+        d = np.abs(ref[r][0] - ref[4][0]).max()
+        print("rows per wave", r, "vs 4:", d)
+        assert d < 2e-6
+
+
+def test_tile_rows_fp64_against_oracle(engine, oracle):
+    """tile_rows regroups the FP64 partial sums only: each choice stays within
+    1e-10 of the FP64 oracle"""
+    sysd = ring_tube(engine)
+    n, steps, B = 600, 40, 3
+    p, sites = make_oracle(oracle, sysd, 16, 1.0)
+    psi0 = np.zeros(n, complex); psi0[0] = 1
+    grp = (np.arange(n) // 150).astype(np.int32)
+    rb = oracle.run_batch(p, sites, sysd["rp"], sysd["col"], sysd["val"], psi0, np.arange(B), steps, 5,
+                          grp, 4)
+    for rows in (64, 128, 256):
+        for rpw in (2, 8):
+            pops, _, _ = _run(engine, sysd, engine.FP64, B,
+                              dict(tile_rows=rows, spmm_rows_per_wave=rpw, spmm_traj_per_lane=1))
+            err = np.abs(pops - rb["pops"]).max()
+            print("tile_rows", rows, "rows/wave", rpw, "err", err)
+            assert err <= 1e-10
+
+
+def test_invalid_tuning_rejected(engine):
+    sysd = ring_tube(engine)
+    s = make_engine(engine, sysd, 16, 1.0, B=64, force_batched=True, precision=engine.FP64)
+    for bad in (dict(spmm_rows_per_wave=3), dict(spmm_traj_per_lane=2), dict(spmm_phases=3),
+                dict(tile_rows=100)):
+        with pytest.raises(engine.DQError):
+            s.set_tuning(**bad)
+    s.close()
+
+
+def test_autotune_keeps_state_and_caches(engine, tmp_path):
+    sysd = ring_tube(engine)
+    n, steps = sysd["n"], 20
+    psi0 = np.zeros(n, complex); psi0[0] = 1
+    grp = (np.arange(n) // 150).astype(np.int32)
+    runs = []
+    for tune in (False, True):
+        s = make_engine(engine, sysd, 16, 1.0, B=128, precision=engine.MIXED)
+        s.set_state(psi0)
+        s.set_observables(grp, 4, 5, 2 * steps)
+        s.run(steps, graph=True)
+        before = s.get_state()
+        if tune:
+            cache = str(tmp_path / "tuning.txt")
+            chosen = s.autotune(reps=1, cache=cache)
+            assert s.tuning() == chosen
+            assert np.array_equal(s.get_state(), before)
+            assert s.get_step() == steps
+            lines = open(cache).read().splitlines()
+            assert len(lines) == 1 and lines[0].endswith(" ".join(str(chosen[k]) for k in engine.TUNING_KEYS))
+            s2 = make_engine(engine, sysd, 16, 1.0, B=128, precision=engine.MIXED)
+            assert s2.autotune(reps=1, cache=cache) == chosen
+            assert len(open(cache).read().splitlines()) == 1
+            s2.close()
+        s.run(steps, graph=True)
+        runs.append((s.populations(), s.get_state()))
+        s.close()
+    # the tuned system continues the same trajectories (tile_rows may regroup FP64 sums)
+    assert np.abs(runs[0][0] - runs[1][0]).max() < 1e-6
