@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch choices (for kernel-trace profiles, whose per-kernel "
+                         "averages would otherwise include the autotuner's trial launches)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the short workload profiled for HBM traffic counters")
     args = ap.parse_args()
@@ -163,6 +166,8 @@ def main():
 
     # ---- headline: batched mixed precision, hipGraph per step
     s = make_system(dq, t, B, dq.MIXED, W + K + args.profile_steps)
+    tuning = s.tuning() if args.no_autotune else s.autotune(reps=2)
+    print("# autotuned launch choices: %s" % tuning, file=sys.stderr, flush=True)
     s.run(W, graph=True)
     s.run(K, graph=True)
     ms = s.last_run_ms()
@@ -230,7 +235,7 @@ def main():
                                "trajectories, m=24, dt=2 fs, 1 Krylov cycle per dt, mixed "
                                "precision, hipGraph per step",
                    "n_sites": n, "nnz_offdiag": nnz, "batch": B, "krylov_m": M, "dt_fs": DT,
-                   "krylov_substeps": 1, "precision": "mixed"},
+                   "krylov_substeps": 1, "precision": "mixed", "tuning": tuning},
         "roofline": roof,
         "spmm_roofline": spmm_roof,
         "plain_fp64_traj_steps_per_s": plain_value,

@@ -52,11 +52,12 @@ class Config(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("spmm_rows_per_wave", C.c_int32), ("spmm_traj_per_lane", C.c_int32),
-        ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("spmm_pipeline", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
-TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows"]
+TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows", "spmm_pipeline"]
 
 
 class TubeSpec(C.Structure):

@@ -215,6 +215,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.rb = 4;
     d.spmm_tpl = (d.mixed && d.Bp % 128 == 0) ? 2 : 1;
     d.spmm_phases = 1;
+    d.spmm_pipe = 0;
     d.rt_s = plain ? 64 : 4 * d.rb;      // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
@@ -1061,6 +1062,10 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "tile_rows must be 64, 128 or 256";
         return false;
     }
+    if (t.spmm_pipeline != 0 && t.spmm_pipeline != 1) {
+        *why = "spmm_pipeline must be 0 or 1";
+        return false;
+    }
     return true;
 }
 
@@ -1071,6 +1076,7 @@ static void get_tuning(const dq_system* sys, dq_tuning* t)
     t->spmm_traj_per_lane = sys->d.spmm_tpl;
     t->spmm_phases = sys->d.spmm_phases;
     t->tile_rows = sys->d.plain ? 0 : sys->d.rt_e;
+    t->spmm_pipeline = sys->d.spmm_pipe;
 }
 
 static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
@@ -1087,6 +1093,7 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     }
     d.spmm_tpl = t.spmm_traj_per_lane;
     d.spmm_phases = t.spmm_phases;
+    d.spmm_pipe = t.spmm_pipeline;
     d.rt_e = t.tile_rows;
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     return DQ_OK;
@@ -1166,8 +1173,8 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 dq_tuning t;
                 std::memset(&t, 0, sizeof(t));
                 std::string why;
-                if (std::sscanf(line + key.size(), "%d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
-                                &t.spmm_phases, &t.tile_rows) == 4 && tuning_valid(sys, t, &why)) {
+                if (std::sscanf(line + key.size(), "%d %d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
+                                &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline) == 5 && tuning_valid(sys, t, &why)) {
                     best = t;
                     hit = true;
                 }
@@ -1193,14 +1200,15 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     for (int R : {4, 2, 8}) {
         for (int tpl : {2, 1}) {
             for (int ph : {1, 2, 4}) {
-                for (int rows : {128, 64, 256}) {
+                for (int cand = 0; cand < 6; ++cand) {
                     if (e != hipSuccess || st != DQ_OK) break;
                     dq_tuning t;
                     std::memset(&t, 0, sizeof(t));
                     t.spmm_rows_per_wave = R;
                     t.spmm_traj_per_lane = tpl;
                     t.spmm_phases = ph;
-                    t.tile_rows = rows;
+                    t.tile_rows = cand % 3 == 0 ? 128 : (cand % 3 == 1 ? 64 : 256);
+                    t.spmm_pipeline = cand / 3;
                     std::string why;
                     if (!tuning_valid(sys, t, &why)) continue;
                     if ((st = apply_tuning(sys, t)) != DQ_OK) break;
@@ -1228,8 +1236,8 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     DQ_TRY(apply_tuning(sys, best));
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
-                         best.spmm_phases, best.tile_rows);
+            std::fprintf(f, "%s %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
+                         best.spmm_phases, best.tile_rows, best.spmm_pipeline);
             std::fclose(f);
         }
     }

@@ -83,7 +83,44 @@ __device__ __forceinline__ double fmav(double a, double b, double c) { return __
 // costs one vector memory instruction addressed as uniform base + lane offset
 // and the vector ALU only runs the FMAs.  TPL trajectories per lane: 2 in FP32
 // (one 16-byte load per lane), 1 in FP64.
+// 8 gathers of x rows rc[p..p+7] (TPL trajectories per lane)
+template <typename VT, int TPL, typename V2>
+__device__ __forceinline__ void gather8(const V2* __restrict__ xs, const int32_t* __restrict__ rc, int p, int Bp,
+                                        int lane, VT (&xr)[8][TPL], VT (&xi)[8][TPL])
+{
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const V2* xc = xs + (size_t)uniform(rc[p + u]) * Bp;
+        if constexpr (TPL == 2) {
+            const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
+            xr[u][0] = q.x; xi[u][0] = q.y; xr[u][1] = q.z; xi[u][1] = q.w;
+        } else {
+            const V2 q = xc[lane];
+            xr[u][0] = q.x; xi[u][0] = q.y;
+        }
+    }
+}
+
+// rows r of the block += value(p+u, r) * x(p+u), in column order
 template <typename VT, int TPL, int kRB>
+__device__ __forceinline__ void fma8(const VT* __restrict__ rv, int p, const VT (&xr)[8][TPL],
+                                     const VT (&xi)[8][TPL], VT (&ar)[TPL][kRB], VT (&ai)[TPL][kRB])
+{
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) {
+            const VT v = rv[(size_t)(p + u) * kRB + r];
+#pragma unroll
+            for (int t = 0; t < TPL; ++t) {
+                ar[t][r] = fmav(v, xr[u][t], ar[t][r]);
+                ai[t][r] = fmav(v, xi[u][t], ai[t][r]);
+            }
+        }
+    }
+}
+
+template <typename VT, int TPL, int kRB, bool PIPE>
 __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int tt0, int ntt)
 {
     using V2 = typename BVec2<VT>::T;
@@ -117,30 +154,30 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
 #pragma unroll
             for (int r = 0; r < kRB; ++r) { ar[t][r] = 0; ai[t][r] = 0; }
         int p = u0;
-        for (; p + 8 <= u1; p += 8) {
-            VT xr[8][TPL], xi[8][TPL];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const V2* xc = xs + (size_t)uniform(rc[p + u]) * Bp;
-                if constexpr (TPL == 2) {
-                    const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
-                    xr[u][0] = q.x; xi[u][0] = q.y; xr[u][1] = q.z; xi[u][1] = q.w;
-                } else {
-                    const V2 q = xc[lane];
-                    xr[u][0] = q.x; xi[u][0] = q.y;
-                }
+        if constexpr (PIPE) {
+            // two register buffers: the gathers of the next 8 columns are in
+            // flight while the current 8 are multiplied in
+            const int nb = (u1 - p) / 8;
+            VT xar[8][TPL], xai[8][TPL], xbr[8][TPL], xbi[8][TPL];
+            if (nb > 0) gather8<VT, TPL>(xs, rc, p, Bp, lane, xar, xai);
+            int bb = 0;
+            for (; bb + 2 <= nb; bb += 2) {
+                gather8<VT, TPL>(xs, rc, p + 8, Bp, lane, xbr, xbi);
+                fma8<VT, TPL, kRB>(rv, p, xar, xai, ar, ai);
+                p += 8;
+                if (bb + 2 < nb) gather8<VT, TPL>(xs, rc, p + 8, Bp, lane, xar, xai);
+                fma8<VT, TPL, kRB>(rv, p, xbr, xbi, ar, ai);
+                p += 8;
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-#pragma unroll
-                for (int r = 0; r < kRB; ++r) {
-                    const VT v = rv[(size_t)(p + u) * kRB + r];
-#pragma unroll
-                    for (int t = 0; t < TPL; ++t) {
-                        ar[t][r] = fmav(v, xr[u][t], ar[t][r]);
-                        ai[t][r] = fmav(v, xi[u][t], ai[t][r]);
-                    }
-                }
+            if (bb < nb) {
+                fma8<VT, TPL, kRB>(rv, p, xar, xai, ar, ai);
+                p += 8;
+            }
+        } else {
+            for (; p + 8 <= u1; p += 8) {
+                VT xr[8][TPL], xi[8][TPL];
+                gather8<VT, TPL>(xs, rc, p, Bp, lane, xr, xi);
+                fma8<VT, TPL, kRB>(rv, p, xr, xi, ar, ai);
             }
         }
         for (; p < u1; ++p) {
@@ -211,14 +248,14 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
     }
 }
 
-template <int RB>
+template <int RB, bool PIPE>
 static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, int ntt, hipStream_t s)
 {
     const dim3 grid(bpad8(d.ntile_s * ntt)), block(kThreads);
     for (int ph = 0; ph < phases; ++ph) {
-        if (tpl == 2) hipLaunchKernelGGL((k_spmm_b<float, 2, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
-        else if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float, 1, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
-        else hipLaunchKernelGGL((k_spmm_b<double, 1, RB>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        if (tpl == 2) hipLaunchKernelGGL((k_spmm_b<float, 2, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        else if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float, 1, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        else hipLaunchKernelGGL((k_spmm_b<double, 1, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
     }
 }
 
@@ -231,10 +268,11 @@ hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
     int phases = d.spmm_phases < 1 ? 1 : d.spmm_phases;
     while (nslice % phases) --phases;
     const int ntt = nslice / phases;
+    const bool pipe = d.spmm_pipe != 0;
     switch (d.rb) {
-    case 2: launch_spmm_rb<2>(d, j, last, tpl, phases, ntt, s); break;
-    case 8: launch_spmm_rb<8>(d, j, last, tpl, phases, ntt, s); break;
-    default: launch_spmm_rb<4>(d, j, last, tpl, phases, ntt, s); break;
+    case 2: pipe ? launch_spmm_rb<2, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<2, false>(d, j, last, tpl, phases, ntt, s); break;
+    case 8: pipe ? launch_spmm_rb<8, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<8, false>(d, j, last, tpl, phases, ntt, s); break;
+    default: pipe ? launch_spmm_rb<4, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<4, false>(d, j, last, tpl, phases, ntt, s); break;
     }
     return hipGetLastError();
 }

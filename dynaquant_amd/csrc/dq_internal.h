@@ -26,6 +26,7 @@ struct Dev {
     int rb;                         // batched SpMM: rows per wavefront (row block)
     int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
     int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
+    int spmm_pipe;                  // batched SpMM: 1 = two gather buffers in flight
     int S;                          // OU substeps
     uint64_t seed;
     double subdt, sqrt_subdt;

@@ -218,7 +218,9 @@ typedef struct dq_tuning {
     int32_t spmm_phases;         /* trajectory slices of one SpMM run as 1, 2 or 4 launches */
     int32_t tile_rows;           /* 64, 128 or 256 rows per workgroup of the update, combine
                                     and noise kernels */
-    int32_t reserved[4];
+    int32_t spmm_pipeline;       /* 0 or 1: gathers of the next 8 union columns in flight while
+                                    the current 8 are multiplied in */
+    int32_t reserved[3];
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

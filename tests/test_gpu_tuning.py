@@ -23,12 +23,12 @@ def _run(engine, sysd, precision, B, tuning, steps=40):
     return out
 
 
-SPMM_CHOICES = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=p)
-                for r in (2, 4, 8) for t in (1, 2) for p in (1, 2)]
+SPMM_CHOICES = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=p, spmm_pipeline=q)
+                for r in (2, 4, 8) for t in (1, 2) for p in (1, 2) for q in (0, 1)]
 
 
 def test_spmm_choices_mixed(engine):
-    """trajectories per lane and phases leave every per-trajectory sum and every
+    """trajectories per lane, phases and the gather pipeline leave every per-trajectory sum and every
     partial-sum tile unchanged: bitwise identical results for one rows-per-wave
     choice.  Rows per wave regroups the FP64 <v|w> tiles, so across it only the
     FP32 rounding noise differs"""
