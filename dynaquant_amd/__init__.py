@@ -257,15 +257,18 @@ class System:
         _check(lib().dq_get_kernel_times(self._h, _p(ms), _p(cnt)))
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
 
-    def adjoint(self, nsteps, ckpt=50, w_pop=None, w_spec=0.0, corr_target=None):
+    def adjoint(self, nsteps, ckpt=50, w_pop=None, w_spec=0.0, corr_target=None, corr_weight=None):
         """(loss[B], dL/d e_site [B, n], dL/dg [B]) by the reverse sweep"""
         wp = None if w_pop is None else _c(w_pop, np.float64)
         ct = None if corr_target is None else c2i(corr_target)
+        cw = None if corr_weight is None else _c(corr_weight, np.float64)
+        if cw is not None and len(cw) != nsteps + 1:
+            raise ValueError("corr_weight needs nsteps + 1 entries")
         loss = np.zeros(self.B)
         ge = np.zeros(self.B * self.n)
         gg = np.zeros(self.B)
         _check(lib().dq_adjoint(self._h, C.c_int64(nsteps), int(ckpt), _p(wp), C.c_double(w_spec),
-                                _p(ct), _p(loss), _p(ge), _p(gg)))
+                                _p(ct), _p(cw), _p(loss), _p(ge), _p(gg)))
         return loss, ge.reshape(self.B, self.n), gg
 
     def get_step(self):

@@ -868,7 +868,8 @@ dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft,
 // adjoint
 // ---------------------------------------------------------------------------
 dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double* w_pop, double w_spec,
-                     const double* corr_target, double* loss, double* grad_e, double* grad_g)
+                     const double* corr_target, const double* corr_weight, double* loss, double* grad_e,
+                     double* grad_g)
 {
     DQ_TRY(check_ready(sys));
     Dev& d = sys->d;
@@ -970,8 +971,9 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
                 const double tr = corr_target ? corr_target[2 * k] : 0.0;
                 const double ti = corr_target ? corr_target[2 * k + 1] : 0.0;
                 const double dr = c.x - tr, di = c.y - ti;
-                L += w_spec * (dr * dr + di * di);
-                cb[(size_t)k * Bp + b] = make_double2(2.0 * w_spec * dr, 2.0 * w_spec * di);
+                const double ws = w_spec * (corr_weight ? corr_weight[k] : 1.0);
+                L += ws * (dr * dr + di * di);
+                cb[(size_t)k * Bp + b] = make_double2(2.0 * ws * dr, 2.0 * ws * di);
             }
         loss[b] = L;
     }

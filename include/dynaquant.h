@@ -179,9 +179,12 @@ dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
 
 /* ---- adjoint ---------------------------------------------------------- */
 /* Gradient of the per-trajectory loss
- *   L_b = sum_g w_pop[g] P_{b,g}(T) + w_spec sum_{n=0..T} |C_b(n) - C_target(n)|^2
- * (groups and bra from dq_system_set_observables; w_pop or C_target may be
- * NULL) over nsteps steps from the current state at step 0, with respect
+ *   L_b = sum_g w_pop[g] P_{b,g}(T) + w_spec sum_{n=0..T} u(n) |C_b(n) - C_target(n)|^2
+ * (groups and bra from dq_system_set_observables; w_pop, C_target and the
+ * weights u = corr_weight[T+1] may be NULL, u = 1).  With u(n) =
+ * nfft (mu2/T)^2 win(n)^2 the second term is the L2 distance of the
+ * windowed, zero-padded spectra of dq_spectrum (Parseval).  The gradient is
+ * taken over nsteps steps from the current state at step 0, with respect
  * to the site energies (grad_e[b][i]) and the coupling scale (grad_g[b]).
  * Exact reverse sweep of the discrete map: states are checkpointed every
  * `ckpt` steps, each segment is recomputed with the noise replayed from
@@ -189,8 +192,8 @@ dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
  * three-term recurrence.  Batched FP64 systems only.  No counterpart in
  * the reference (it has no derivatives). */
 dq_status dq_adjoint(dq_system *sys, int64_t nsteps, int32_t ckpt, const double *w_pop,
-                     double w_spec, const double *corr_target, double *loss, double *grad_e,
-                     double *grad_g);
+                     double w_spec, const double *corr_target, const double *corr_weight,
+                     double *loss, double *grad_e, double *grad_g);
 
 /* ---- timing ----------------------------------------------------------- */
 /* Milliseconds of the last dq_run, measured with HIP events on the

@@ -15,7 +15,7 @@
  *     (systemClass.cu:1769-1829).
  * The oracle's GRAM projections equal the CGS ones in exact arithmetic, so
  * this is the derivative of both.  Loss:
- *   L = sum_g w_pop[g] P_g(T) + w_spec sum_{n=0..T} |C(n) - C_target(n)|^2,
+ *   L = sum_g w_pop[g] P_g(T) + w_spec sum_{n=0..T} u(n) |C(n) - C_target(n)|^2,
  *   P_g = sum_{i in g} |psi_i|^2,  C(n) = <bra|psi(n)>.
  * Parameters: the site energies (additive on the diagonal at every step)
  * and the coupling scale g.  States are checkpointed every `ckpt` steps;
@@ -532,7 +532,7 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
                    const double *psi0, uint64_t traj, int64_t nsteps,
                    const int32_t *group, int ngroups, const double *w_pop,
                    const double *bra, double w_spec, const double *corr_target,
-                   int ckpt, double *grad_e, double *grad_g)
+                   const double *corr_weight, int ckpt, double *grad_e, double *grad_g)
 {
     const int n = p->n, m = p->m, K = p->krylov_substeps;
     const size_t cs = 2 * (size_t)n;
@@ -591,9 +591,10 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
         for (int64_t s = 0; s <= nsteps; ++s) {
             double dr = corr[2 * s] - (corr_target ? corr_target[2 * s] : 0.0);
             double di = corr[2 * s + 1] - (corr_target ? corr_target[2 * s + 1] : 0.0);
-            loss += w_spec * (dr * dr + di * di);
-            cbar[2 * s] = 2.0 * w_spec * dr;
-            cbar[2 * s + 1] = 2.0 * w_spec * di;
+            const double ws = w_spec * (corr_weight ? corr_weight[s] : 1.0);
+            loss += ws * (dr * dr + di * di);
+            cbar[2 * s] = 2.0 * ws * dr;
+            cbar[2 * s + 1] = 2.0 * ws * di;
         }
     }
     memset(grad_e, 0, sizeof(double) * n);

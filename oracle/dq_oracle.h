@@ -161,8 +161,8 @@ int orc_spectrum(int T, const double *corr, double dt_fs, double sigma_fs,
                  double mu2, int nfft, double *omega, double *spec);
 
 /* ---- adjoint --------------------------------------------------------- */
-/* Loss = sum_g w_pop[g] * P_g(T) + w_spec * sum_n |C(n) - C_target(n)|^2
- * on one trajectory.  Gradient w.r.t. the n site energies and g.
+/* Loss = sum_g w_pop[g] * P_g(T) + w_spec * sum_n u(n) |C(n) - C_target(n)|^2
+ * on one trajectory (u = corr_weight, NULL: 1).  Gradient w.r.t. the n site energies and g.
  * Exact reverse mode through the discrete noise/Lanczos/Pade map with
  * checkpoints every `ckpt` steps.  Returns loss. */
 double orc_adjoint(const orc_params *p, const orc_site_params *sp,
@@ -170,7 +170,7 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
                    const double *psi0, uint64_t traj, int64_t nsteps,
                    const int32_t *group, int ngroups, const double *w_pop,
                    const double *bra, double w_spec, const double *corr_target,
-                   int ckpt, double *grad_e, double *grad_g);
+                   const double *corr_weight, int ckpt, double *grad_e, double *grad_g);
 
 /* 0: reverse of the full-projection Lanczos as computed (O(m^2) vector ops);
  * 1 (default): reverse of the equivalent three-term recurrence on the

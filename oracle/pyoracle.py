@@ -99,7 +99,7 @@ def lib():
         L.orc_adjoint.restype = C.c_double
         L.orc_adjoint.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), _ip, _ip, _dp, _dp,
                                   C.c_uint64, C.c_int64, C.c_void_p, C.c_int, C.c_void_p,
-                                  C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
+                                  C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, _dp,
                                   C.POINTER(C.c_double)]
         L.orc_set_adjoint_mode.argtypes = [C.c_int]
         _lib = L
@@ -332,23 +332,24 @@ def spectrum(corr, dt, sigma, mu2, nfft):
 
 
 def adjoint(p, sites, rowptr, col, val, psi0, traj, nsteps, group=None, ngroups=0, w_pop=None,
-            bra=None, w_spec=0.0, corr_target=None, ckpt=50):
+            bra=None, w_spec=0.0, corr_target=None, ckpt=50, corr_weight=None):
     """(loss, dL/d e_site[n], dL/dg) by the exact reverse sweep"""
     n = p.n
     grp = None if group is None else _c(np.asarray(group, np.int32))
     wp = None if w_pop is None else _c(np.asarray(w_pop, np.float64))
     braa = None if bra is None else c2i(bra)
     ct = None if corr_target is None else c2i(corr_target)
+    cw = None if corr_weight is None else _c(np.asarray(corr_weight, np.float64))
     ge = np.zeros(n)
     gg = C.c_double()
     loss = lib().orc_adjoint(C.byref(p), C.byref(sites.sp), _c(rowptr), _c(col), _c(val), c2i(psi0),
                              traj, nsteps, _ptr(grp), ngroups, _ptr(wp), _ptr(braa), w_spec, _ptr(ct),
-                             ckpt, ge, C.byref(gg))
+                             _ptr(cw), ckpt, ge, C.byref(gg))
     return loss, ge, gg.value
 
 
 def loss_fd(p, sites_kw, rowptr, col, val, psi0, traj, nsteps, group, ngroups, w_pop,
-            bra=None, w_spec=0.0, corr_target=None, e_shift=None):
+            bra=None, w_spec=0.0, corr_target=None, e_shift=None, corr_weight=None):
     """the same loss evaluated by orc_run (for finite differences)"""
     n = p.n
     s = Sites(n, e_shift=e_shift, **sites_kw)
@@ -357,5 +358,6 @@ def loss_fd(p, sites_kw, rowptr, col, val, psi0, traj, nsteps, group, ngroups, w
     L = float(np.dot(w_pop, r["pops"][-1])) if w_pop is not None else 0.0
     if bra is not None and w_spec != 0.0:
         ct = 0.0 if corr_target is None else np.asarray(corr_target)
-        L += w_spec * float(np.sum(np.abs(r["corr"] - ct) ** 2))
+        u = 1.0 if corr_weight is None else np.asarray(corr_weight)
+        L += w_spec * float(np.sum(u * np.abs(r["corr"] - ct) ** 2))
     return L

@@ -85,3 +85,51 @@ def test_three_term_reverse_equals_full_projection_reverse(oracle, small):
     b = oracle.adjoint(p, sites, *args, bra=s["bra"], w_spec=0.2)
     assert np.abs(a[1] - b[1]).max() < 1e-11 * np.abs(a[1]).max()
     assert abs(a[2] - b[2]) < 1e-11 * abs(a[2])
+
+
+def spectrum_weights(T, dt, sigma_fs, mu2, nfft):
+    """u(n) such that sum_n u(n)|C(n) - C_t(n)|^2 is the L2 distance of the
+    zero-padded windowed spectra S = (mu2/T) FFT(win * C): Parseval for an
+    unnormalised DFT of length nfft gives sum_k |S_k|^2 = nfft sum_n |s_n|^2"""
+    n = np.arange(T + 1)
+    # window at t = (n+1) dt for n < T, none on the last sample (spectraClass.cu:591-598)
+    win = np.where(n < T, np.exp(-0.5 * ((n + 1) * dt / sigma_fs) ** 2), 1.0)
+    return nfft * (mu2 / T) ** 2 * win ** 2
+
+
+def test_weighted_correlation_loss_is_the_spectrum_distance(oracle, small):
+    """the time-domain weights turn the C(t) term into the L2 mismatch of the
+    windowed, zero-padded spectra computed by the oracle's own spectrum"""
+    s = small
+    n, T, dt, nfft = s["n"], 40, 1.0, 128
+    p = oracle.make_params(n, 8, dt, sigma_static=200.0, seed=1, reorth=1)
+    run = oracle.run(p, oracle.Sites(n, tau=50.0, sigma=300.0, e_site=0.0), s["rp"], s["col"], s["val"],
+                     s["psi0"], 0, T, obs_every=T, bra=s["bra"])
+    shifted = oracle.run(p, oracle.Sites(n, tau=50.0, sigma=300.0, e_site=40.0), s["rp"], s["col"], s["val"],
+                         s["psi0"], 0, T, obs_every=T, bra=s["bra"])
+    u = spectrum_weights(T, dt, 20.0, 1.0, nfft)
+    direct = float(np.sum(u * np.abs(run["corr"] - shifted["corr"]) ** 2))
+    _, sa = oracle.spectrum(run["corr"], dt, 20.0, 1.0, nfft)
+    _, sb = oracle.spectrum(shifted["corr"], dt, 20.0, 1.0, nfft)
+    spec = float(np.sum(np.abs(sa - sb) ** 2))
+    assert direct == pytest.approx(spec, rel=1e-12)
+
+
+def test_weighted_spectrum_loss_gradient_matches_central_differences(oracle, small):
+    s = small
+    n, T, dt = s["n"], 30, 1.0
+    p = oracle.make_params(n, 8, dt, sigma_static=200.0, seed=1, reorth=1)
+    kw = dict(tau=50.0, sigma=300.0, e_site=0.0)
+    target = oracle.run(p, oracle.Sites(n, tau=50.0, sigma=300.0, e_site=25.0), s["rp"], s["col"],
+                        s["val"], s["psi0"], 0, T, obs_every=T, bra=s["bra"])["corr"]
+    u = spectrum_weights(T, dt, 15.0, 1.0, 64)
+    args = (s["rp"], s["col"], s["val"], s["psi0"], 0, T, s["grp"], 2, [0.0, 1.0])
+    opt = dict(bra=s["bra"], w_spec=1e-3, corr_target=target, corr_weight=u)
+    loss, ge, gg = oracle.adjoint(p, oracle.Sites(n, **kw), *args, ckpt=7, **opt)
+    assert loss == pytest.approx(oracle.loss_fd(p, kw, *args, **opt), rel=1e-13)
+    h = 1e-3
+    for i in (0, 50, 170):
+        e = np.zeros(n); e[i] = h
+        fd = (oracle.loss_fd(p, kw, *args, e_shift=e, **opt)
+              - oracle.loss_fd(p, kw, *args, e_shift=-e, **opt)) / (2 * h)
+        assert abs(ge[i] - fd) <= 1e-5 * abs(fd) + 1e-12
