@@ -26,7 +26,6 @@ struct AdjDev {
     double2* y;                // [m][Bp] Pade first column
     double* abar;              // [m][Bp]
     double* bbar;              // [m][Bp]
-    double* r1;                // [Bp]
     double* r2;                // [Bp]
     double* r3;                // [m][Bp]
     double2* cbar;             // [T+1][Bp] adjoint of C(n)

@@ -79,46 +79,63 @@ __global__ __launch_bounds__(kThreads) void k_adj_renorm_dot(Dev d, AdjDev a, in
     adj_store_partials<1>(acc, 1, d.part, d.ntile_e, t.rtile, d.Bp, t.b);
 }
 
-// part 2: psibar' = (pbar - psi'' r)/|psi'|, partial ybar_k = <q_k|psibar'>
-template <int MAXV>
+// part 2: psibar' = (pbar - psi'' r)/|psi'|, partial ybar_k = <q_k|psibar'>.
+// Lane = 16 kg + t: 16 trajectories per workgroup, lane group kg owns the
+// basis vectors k = kg + 4u, so each lane keeps 2 KW partial sums and no
+// shuffles are needed; the four waves (rows) meet in LDS at the end.
+template <int KW>
 __global__ __launch_bounds__(kThreads) void k_adj_renorm_apply(Dev d, AdjDev a, int c)
 {
-    ATile t = adj_tile(d, d.rt_e, d.ntile_e);
-    if (!t.ok) return;
-    const int b = t.b, Bp = d.Bp, m = d.m;
+    __shared__ double lds[4][2 * KW][64];
+    const int ntt = d.Bp / 16;
+    const int logical = xcd_tile(blockIdx.x, gridDim.x);
+    if (logical >= d.ntile_e * ntt) return;
+    const int rtile = logical / ntt, ttile = logical % ntt;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = lane & 15, kg = lane >> 4;
+    const int b = ttile * 16 + t;
+    const int Bp = d.Bp, m = d.m;
     const size_t nB = (size_t)d.n * Bp;
     const double r = d.red[b];
     const double inv = 1.0 / __builtin_sqrt(a.nrm2[(size_t)c * Bp + b]);
     const double2* out = a.psi_out(c);
     const double2* V = a.basis(c);
     const double* sc = a.sc + (size_t)c * (m + 1) * Bp;
-    double acc[2 * MAXV];
+    double acc[2 * KW];
 #pragma unroll
-    for (int q = 0; q < 2 * MAXV; ++q) acc[q] = 0.0;
-    for (int i = t.rfirst; i < t.r1; i += 4) {
+    for (int q = 0; q < 2 * KW; ++q) acc[q] = 0.0;
+    const int r0 = rtile * d.rt_e, r1 = min(d.n, r0 + d.rt_e);
+    for (int i = r0 + wave; i < r1; i += 4) {
         const size_t e = (size_t)i * Bp + b;
         const double2 p = out[e], pb = a.pbar[e];
         const double2 v = make_double2((pb.x - p.x * r) * inv, (pb.y - p.y * r) * inv);
-        a.pbar1[e] = v;
+        if (kg == 0) a.pbar1[e] = v;
 #pragma unroll
-        for (int k = 0; k < MAXV; ++k) {
+        for (int u = 0; u < KW; ++u) {
+            const int k = kg + 4 * u;
             if (k < m) {
                 const double2 q = V[(size_t)k * nB + e];
-                acc[2 * k] += q.x * v.x + q.y * v.y;
-                acc[2 * k + 1] += q.x * v.y - q.y * v.x;
+                acc[2 * u] += q.x * v.x + q.y * v.y;
+                acc[2 * u + 1] += q.x * v.y - q.y * v.x;
             }
         }
     }
-    // scale by s_k (q_k = s_k v_k)
 #pragma unroll
-    for (int k = 0; k < MAXV; ++k) {
-        if (k < m) {
-            const double s = sc[(size_t)k * Bp + b];
-            acc[2 * k] *= s;
-            acc[2 * k + 1] *= s;
+    for (int q = 0; q < 2 * KW; ++q) lds[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int u = 0; u < KW; ++u) {
+            const int k = kg + 4 * u;
+            if (k >= m) continue;
+            const double sk = sc[(size_t)k * Bp + b];        // q_k = s_k v_k
+            for (int cc = 0; cc < 2; ++cc) {
+                const int q = 2 * u + cc;
+                const double v = (lds[0][q][lane] + lds[1][q][lane]) + (lds[2][q][lane] + lds[3][q][lane]);
+                d.part[((size_t)(2 * k + cc) * d.ntile_e + rtile) * Bp + b] = sk * v;
+            }
         }
     }
-    adj_store_partials<2 * MAXV>(acc, 2 * m, d.part, d.ntile_e, t.rtile, Bp, b);
 }
 
 // ---------------------------------------------------------------------------
@@ -379,7 +396,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_wbar(Dev d, AdjDev a, int c, i
     // betabar_j: from the Pade plus -r3 of iteration j+1; r1 = Re<qbar_{j+1}|q_{j+1}>
     double coef_q = 0.0, coef_qb = 0.0;
     if (bj > 0.0 && j + 1 < m) {
-        const double btot = a.bbar[(size_t)j * Bp + b] - a.r3[(size_t)(j + 1) * Bp + b] - a.r1[b] / bj;
+        // r1 = Re<qbar_{j+1}|q_{j+1}>: the last reduction (after the SpMM
+        // reverse of iteration j+1), still in red[0..Bp)
+        const double r1 = d.red[b];
+        const double btot = a.bbar[(size_t)j * Bp + b] - a.r3[(size_t)(j + 1) * Bp + b] - r1 / bj;
         coef_qb = 1.0 / bj;
         coef_q = btot * sj1;
     }
@@ -500,12 +520,155 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm(Dev d, AdjDev a, int c, i
     adj_store_partials<1>(acc, 1, d.part, d.ntile_s, rtile, Bp, b);
 }
 
-__global__ void k_adj_store_r1(Dev d, AdjDev a)
+// Row-blocked, fused form of k_adj_wbar + k_adj_spmm (g != 0).  A wavefront
+// walks the union columns of kRB rows and gathers qbar_{j+1} only:
+//   wbar' = qbar_{j+1}/beta_j + btot q_{j+1}   (btot uses r1 = Re<qbar_{j+1}|q_{j+1}>),
+//   J wbar' = J qbar_{j+1}/beta_j + btot J q_{j+1},
+// with J q_k = (H q_k - d q_k)/g from the three-term relation
+// H q_k = beta_k q_{k+1} + alpha_k q_k + beta_{k-1} q_{k-1}, the relation the
+// reverse already uses for alphabar (H q_j).  r2 = Re<q_j|wbar'> and
+// r3 = Re<q_{j-1}|wbar'> follow from five sums that iteration j+1 left in the
+// reduction buffer (red[0..5): Re<qbar_{j+1}|q_{j+1}>, Re<q_j|qbar_{j+1}>,
+// Re<q_j|q_{j+1}>, Re<q_{j-1}|qbar_{j+1}>, Re<q_{j-1}|q_{j+1}>); this kernel
+// writes the same five partial sums for iteration j-1.  Rows at or beyond a
+// trajectory's n_active take no update, as the forward SpMM writes 0 there.
+template <int kRB>
+__global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c, int j)
 {
-    for (int b = threadIdx.x + blockIdx.x * blockDim.x; b < d.Bp; b += blockDim.x * gridDim.x) a.r1[b] = d.red[b];
+    const int ntt = d.Bp / 64;
+    const int logical = xcd_tile(blockIdx.x, gridDim.x);
+    if (logical >= d.ntile_s * ntt) return;
+    const int rtile = logical / ntt;
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int slice = logical % ntt;
+    const int b = slice * 64 + lane;
+    const int Bp = d.Bp, m = d.m;
+    const size_t nB = (size_t)d.n * Bp;
+    const double2* V = a.basis(c);
+    const double* sc = a.sc + (size_t)c * (m + 1) * Bp;
+    const double* al = a.alpha + (size_t)c * m * Bp;
+    const double* be = a.beta + (size_t)c * m * Bp;
+    const double* diag = a.diag_step(c);
+    auto S = [&](int k) { return (k >= 0 && k <= m) ? sc[(size_t)k * Bp + b] : 0.0; };
+    auto A = [&](int k) { return (k >= 0 && k < m) ? al[(size_t)k * Bp + b] : 0.0; };
+    auto Bt = [&](int k) { return (k >= 0 && k < m) ? be[(size_t)k * Bp + b] : 0.0; };
+    const double sj = S(j), sj1 = S(j + 1), sj2 = S(j + 2), sjm = S(j - 1), sjm2 = S(j - 2);
+    const double aj = A(j), aj1 = A(j + 1);
+    const double bj = Bt(j), bj1 = Bt(j + 1), bjm = Bt(j - 1);
+    // wbar' = cqb qbar_{j+1} + btot q_{j+1}
+    const bool top = !(bj > 0.0 && j + 1 < m);
+    double cqb = 0.0, btot = 0.0, r2 = 0.0, r3 = 0.0;
+    if (!top) {
+        const double R0 = d.red[b], R1 = d.red[(size_t)Bp + b], R2 = d.red[(size_t)2 * Bp + b];
+        const double R3 = d.red[(size_t)3 * Bp + b], R4 = d.red[(size_t)4 * Bp + b];
+        cqb = 1.0 / bj;
+        btot = a.bbar[(size_t)j * Bp + b] - a.r3[(size_t)(j + 1) * Bp + b] - R0 / bj;
+        r2 = cqb * R1 + btot * R2;
+        r3 = cqb * R3 + btot * R4;
+    }
+    if (rtile == 0 && wave == 0) a.r3[(size_t)j * Bp + b] = r3;
+    const double abart = a.abar[(size_t)j * Bp + b] - r2;
+    const double2 yjm = j >= 1 ? a.y[(size_t)(j - 1) * Bp + b] : make_double2(0.0, 0.0);
+    double2* qbj = a.qbar_slot(j);
+    double2* qbm = j >= 1 ? a.qbar_slot(j - 1) : nullptr;
+    const double2* qb1 = a.qbar_slot(j + 1);
+    const double g = d.g, ginv = 1.0 / d.g;
+    const int nact = d.nact[b];
+    const double2* __restrict__ xs = qb1 + (size_t)slice * 64;
+    const double* __restrict__ rv = d.rb_val64;
+    const int32_t* __restrict__ rc = d.rb_col;
+    const int nblk = (d.n + kRB - 1) / kRB;
+    const int rb = rtile * 4 + wave;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // R0'..R4' for j-1, gbar
+    if (rb < nblk) {
+        double sr[kRB], si[kRB];
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { sr[r] = 0.0; si[r] = 0.0; }
+        if (!uniform((int)top)) {
+            const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+            int p = u0;
+            for (; p + 8 <= u1; p += 8) {
+                double2 xw[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) xw[u] = xs[(size_t)uniform(rc[p + u]) * Bp + lane];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+#pragma unroll
+                    for (int r = 0; r < kRB; ++r) {
+                        const double v = rv[(size_t)(p + u) * kRB + r];
+                        sr[r] = __builtin_fma(v, xw[u].x, sr[r]);
+                        si[r] = __builtin_fma(v, xw[u].y, si[r]);
+                    }
+                }
+            }
+            for (; p < u1; ++p) {
+                const double2 xw = xs[(size_t)uniform(rc[p]) * Bp + lane];
+#pragma unroll
+                for (int r = 0; r < kRB; ++r) {
+                    const double v = rv[(size_t)p * kRB + r];
+                    sr[r] = __builtin_fma(v, xw.x, sr[r]);
+                    si[r] = __builtin_fma(v, xw.y, si[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) {
+            const int i = rb * kRB + r;
+            if (i >= d.n || i >= nact) continue;
+            const size_t e = (size_t)i * Bp + b;
+            const double di = diag[e];
+            const double2 xj = V[(size_t)j * nB + e];
+            const double2 q = make_double2(sj * xj.x, sj * xj.y);
+            const double2 x1 = V[(size_t)(j + 1) * nB + e];
+            const double2 q1 = make_double2(sj1 * x1.x, sj1 * x1.y);
+            double2 qm = make_double2(0.0, 0.0), qm2 = make_double2(0.0, 0.0);
+            if (j >= 1) { const double2 t = V[(size_t)(j - 1) * nB + e]; qm = make_double2(sjm * t.x, sjm * t.y); }
+            if (j >= 2) { const double2 t = V[(size_t)(j - 2) * nB + e]; qm2 = make_double2(sjm2 * t.x, sjm2 * t.y); }
+            // H q_j by the three-term relation
+            const double2 hq = make_double2(bj * q1.x + aj * q.x + bjm * qm.x, bj * q1.y + aj * q.y + bjm * qm.y);
+            double2 wp = make_double2(0.0, 0.0), Jw = make_double2(0.0, 0.0);
+            if (!top) {
+                const double2 qbn = qb1[e];
+                wp = make_double2(cqb * qbn.x + btot * q1.x, cqb * qbn.y + btot * q1.y);
+                // H q_{j+1} by the three-term relation (slot j+2 <= m holds q_{j+2})
+                const double2 x2 = V[(size_t)(j + 2) * nB + e];
+                const double2 hq1 = make_double2(bj1 * sj2 * x2.x + aj1 * q1.x + bj * q.x,
+                                                 bj1 * sj2 * x2.y + aj1 * q1.y + bj * q.y);
+                Jw = make_double2(cqb * sr[r] + btot * (hq1.x - di * q1.x) * ginv,
+                                  cqb * si[r] + btot * (hq1.y - di * q1.y) * ginv);
+            }
+            const double2 wbar = make_double2(wp.x + abart * q.x, wp.y + abart * q.y);
+            // J wbar = J wbar' + alphabar J q_j
+            Jw.x += abart * (hq.x - di * q.x) * ginv;
+            Jw.y += abart * (hq.y - di * q.y) * ginv;
+            const double2 Hw = make_double2(di * wbar.x + g * Jw.x, di * wbar.y + g * Jw.y);
+            double2 qb = qbj[e];
+            qb.x += -aj * wp.x + abart * hq.x + Hw.x;
+            qb.y += -aj * wp.y + abart * hq.y + Hw.y;
+            qbj[e] = qb;
+            if (j >= 1) {
+                const double2 pb = a.pbar1[e];
+                qbm[e] = make_double2(yjm.x * pb.x + yjm.y * pb.y - bjm * wp.x, yjm.x * pb.y - yjm.y * pb.x - bjm * wp.y);
+            }
+            a.grad_e[e] += wbar.x * q.x + wbar.y * q.y;
+            acc[0] += qb.x * q.x + qb.y * q.y;              // Re<qbar_j|q_j>
+            acc[1] += qm.x * qb.x + qm.y * qb.y;            // Re<q_{j-1}|qbar_j>
+            acc[2] += qm.x * q.x + qm.y * q.y;              // Re<q_{j-1}|q_j>
+            acc[3] += qm2.x * qb.x + qm2.y * qb.y;          // Re<q_{j-2}|qbar_j>
+            acc[4] += qm2.x * q.x + qm2.y * q.y;            // Re<q_{j-2}|q_j>
+            acc[5] += Jw.x * q.x + Jw.y * q.y;              // Re<J wbar|q_j>
+        }
+    }
+    __shared__ double lds[4][64];
+    __syncthreads();
+    lds[wave][lane] = acc[5];
+    __syncthreads();
+    if (wave == 0) a.gpart[(size_t)rtile * Bp + b] += (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
+    adj_store_partials<5>(*reinterpret_cast<const double(*)[5]>(acc), 5, d.part, d.ntile_s, rtile, Bp, b);
 }
 
-// qbar_{m-1} = conj(y_{m-1}) psibar' (start of the three-term reverse), r1 = 0
+// qbar_{m-1} = conj(y_{m-1}) psibar' (start of the three-term reverse)
 __global__ void k_adj_init_top(Dev d, AdjDev a)
 {
     const int m = d.m, Bp = d.Bp;
@@ -515,7 +678,6 @@ __global__ void k_adj_init_top(Dev d, AdjDev a)
         const int b = (int)(e % Bp);
         const double2 y = a.y[(size_t)(m - 1) * Bp + b], pb = a.pbar1[e];
         qb[e] = make_double2(y.x * pb.x + y.y * pb.y, y.x * pb.y - y.y * pb.x);
-        if (e < (size_t)Bp) a.r1[e] = 0.0;
     }
 }
 
@@ -542,10 +704,11 @@ hipError_t adj_renorm(const Dev& d, const AdjDev& a, int c, hipStream_t s)
     hipLaunchKernelGGL(k_adj_renorm_dot, grid, block, 0, s, d, a, c);
     hipError_t e = launch_reduce(d, 1, d.ntile_e, s);
     if (e != hipSuccess) return e;
-    if (d.m <= 8) hipLaunchKernelGGL((k_adj_renorm_apply<8>), grid, block, 0, s, d, a, c);
-    else if (d.m <= 16) hipLaunchKernelGGL((k_adj_renorm_apply<16>), grid, block, 0, s, d, a, c);
-    else if (d.m <= 24) hipLaunchKernelGGL((k_adj_renorm_apply<24>), grid, block, 0, s, d, a, c);
-    else hipLaunchKernelGGL((k_adj_renorm_apply<32>), grid, block, 0, s, d, a, c);
+    const dim3 g16(pad8a(d.ntile_e * (d.Bp / 16)));
+    if (d.m <= 8) hipLaunchKernelGGL((k_adj_renorm_apply<2>), g16, block, 0, s, d, a, c);
+    else if (d.m <= 16) hipLaunchKernelGGL((k_adj_renorm_apply<4>), g16, block, 0, s, d, a, c);
+    else if (d.m <= 24) hipLaunchKernelGGL((k_adj_renorm_apply<6>), g16, block, 0, s, d, a, c);
+    else hipLaunchKernelGGL((k_adj_renorm_apply<8>), g16, block, 0, s, d, a, c);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return launch_reduce(d, 2 * d.m, d.ntile_e, s);
 }
@@ -564,13 +727,24 @@ hipError_t adj_lanczos(const Dev& d, const AdjDev& a, int c, hipStream_t s)
     int gi = (int)std::min<size_t>((nB + kThreads - 1) / kThreads, 65535);
     hipLaunchKernelGGL(k_adj_init_top, dim3(gi), block, 0, s, d, a);
     hipError_t e;
+    if (d.g != 0.0) {
+        // fused: one gather kernel and one reduction per iteration
+        for (int j = d.m - 1; j >= 0; --j) {
+            switch (d.rb) {
+            case 2: hipLaunchKernelGGL((k_adj_spmm_rb<2>), gs, block, 0, s, d, a, c, j); break;
+            case 8: hipLaunchKernelGGL((k_adj_spmm_rb<8>), gs, block, 0, s, d, a, c, j); break;
+            default: hipLaunchKernelGGL((k_adj_spmm_rb<4>), gs, block, 0, s, d, a, c, j); break;
+            }
+            if ((e = launch_reduce(d, 5, d.ntile_s, s)) != hipSuccess) return e;
+        }
+        return hipGetLastError();
+    }
     for (int j = d.m - 1; j >= 0; --j) {
         hipLaunchKernelGGL(k_adj_wbar, ge, block, 0, s, d, a, c, j);
         if ((e = launch_reduce(d, 2, d.ntile_e, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_adj_store_r23, dim3(1), block, 0, s, d, a, j);
         hipLaunchKernelGGL(k_adj_spmm, gs, block, 0, s, d, a, c, j);
         if ((e = launch_reduce(d, 1, d.ntile_s, s)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_adj_store_r1, dim3(1), block, 0, s, d, a);
     }
     return hipGetLastError();
 }

@@ -922,7 +922,6 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     AGET(&a.y, sizeof(double2) * Bp * m);
     AGET(&a.abar, sizeof(double) * Bp * m);
     AGET(&a.bbar, sizeof(double) * Bp * m);
-    AGET(&a.r1, sizeof(double) * Bp);
     AGET(&a.r2, sizeof(double) * Bp);
     AGET(&a.r3, sizeof(double) * Bp * m);
     AGET(&a.cbar, sizeof(double2) * Bp * (nsteps + 1));
