@@ -522,16 +522,18 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm(Dev d, AdjDev a, int c, i
 
 // Row-blocked, fused form of k_adj_wbar + k_adj_spmm (g != 0).  A wavefront
 // walks the union columns of kRB rows and gathers qbar_{j+1} only:
-//   wbar' = qbar_{j+1}/beta_j + btot q_{j+1}   (btot uses r1 = Re<qbar_{j+1}|q_{j+1}>),
+//   wbar' = qbar_{j+1}/beta_j + btot q_{j+1}
+//     (btot = betabar_j - Re<q_j|wbar'_{j+1}> - Re<qbar_{j+1}|q_{j+1}>/beta_j),
 //   J wbar' = J qbar_{j+1}/beta_j + btot J q_{j+1},
 // with J q_k = (H q_k - d q_k)/g from the three-term relation
 // H q_k = beta_k q_{k+1} + alpha_k q_k + beta_{k-1} q_{k-1}, the relation the
-// reverse already uses for alphabar (H q_j).  r2 = Re<q_j|wbar'> and
-// r3 = Re<q_{j-1}|wbar'> follow from five sums that iteration j+1 left in the
-// reduction buffer (red[0..5): Re<qbar_{j+1}|q_{j+1}>, Re<q_j|qbar_{j+1}>,
-// Re<q_j|q_{j+1}>, Re<q_{j-1}|qbar_{j+1}>, Re<q_{j-1}|q_{j+1}>); this kernel
-// writes the same five partial sums for iteration j-1.  Rows at or beyond a
-// trajectory's n_active take no update, as the forward SpMM writes 0 there.
+// reverse already uses for alphabar (H q_j).  The global sums this needs come
+// from the previous iteration's reduction, red[0..4):
+//   Re<qbar_{j+1}|q_{j+1}>, Re<q_j|qbar_{j+1}>, Re<q_j|q_{j+1}>, Re<q_j|wbar'_{j+1}>
+// (so r2 = Re<q_j|wbar'> = Re<q_j|qbar_{j+1}>/beta_j + btot Re<q_j|q_{j+1}>),
+// and this kernel writes the same four partial sums for iteration j-1.  Rows
+// at or beyond a trajectory's n_active take no update, as the forward SpMM
+// writes 0 there.
 template <int kRB>
 __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c, int j)
 {
@@ -553,21 +555,19 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
     auto S = [&](int k) { return (k >= 0 && k <= m) ? sc[(size_t)k * Bp + b] : 0.0; };
     auto A = [&](int k) { return (k >= 0 && k < m) ? al[(size_t)k * Bp + b] : 0.0; };
     auto Bt = [&](int k) { return (k >= 0 && k < m) ? be[(size_t)k * Bp + b] : 0.0; };
-    const double sj = S(j), sj1 = S(j + 1), sj2 = S(j + 2), sjm = S(j - 1), sjm2 = S(j - 2);
+    const double sj = S(j), sj1 = S(j + 1), sj2 = S(j + 2), sjm = S(j - 1);
     const double aj = A(j), aj1 = A(j + 1);
     const double bj = Bt(j), bj1 = Bt(j + 1), bjm = Bt(j - 1);
     // wbar' = cqb qbar_{j+1} + btot q_{j+1}
     const bool top = !(bj > 0.0 && j + 1 < m);
-    double cqb = 0.0, btot = 0.0, r2 = 0.0, r3 = 0.0;
+    double cqb = 0.0, btot = 0.0, r2 = 0.0;
     if (!top) {
         const double R0 = d.red[b], R1 = d.red[(size_t)Bp + b], R2 = d.red[(size_t)2 * Bp + b];
-        const double R3 = d.red[(size_t)3 * Bp + b], R4 = d.red[(size_t)4 * Bp + b];
+        const double R3 = d.red[(size_t)3 * Bp + b];
         cqb = 1.0 / bj;
-        btot = a.bbar[(size_t)j * Bp + b] - a.r3[(size_t)(j + 1) * Bp + b] - R0 / bj;
+        btot = a.bbar[(size_t)j * Bp + b] - R3 - R0 / bj;
         r2 = cqb * R1 + btot * R2;
-        r3 = cqb * R3 + btot * R4;
     }
-    if (rtile == 0 && wave == 0) a.r3[(size_t)j * Bp + b] = r3;
     const double abart = a.abar[(size_t)j * Bp + b] - r2;
     const double2 yjm = j >= 1 ? a.y[(size_t)(j - 1) * Bp + b] : make_double2(0.0, 0.0);
     double2* qbj = a.qbar_slot(j);
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
     const int32_t* __restrict__ rc = d.rb_col;
     const int nblk = (d.n + kRB - 1) / kRB;
     const int rb = rtile * 4 + wave;
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // R0'..R4' for j-1, gbar
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};   // the four sums for j-1, gbar
     if (rb < nblk) {
         double sr[kRB], si[kRB];
 #pragma unroll
@@ -622,9 +622,8 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
             const double2 q = make_double2(sj * xj.x, sj * xj.y);
             const double2 x1 = V[(size_t)(j + 1) * nB + e];
             const double2 q1 = make_double2(sj1 * x1.x, sj1 * x1.y);
-            double2 qm = make_double2(0.0, 0.0), qm2 = make_double2(0.0, 0.0);
+            double2 qm = make_double2(0.0, 0.0);
             if (j >= 1) { const double2 t = V[(size_t)(j - 1) * nB + e]; qm = make_double2(sjm * t.x, sjm * t.y); }
-            if (j >= 2) { const double2 t = V[(size_t)(j - 2) * nB + e]; qm2 = make_double2(sjm2 * t.x, sjm2 * t.y); }
             // H q_j by the three-term relation
             const double2 hq = make_double2(bj * q1.x + aj * q.x + bjm * qm.x, bj * q1.y + aj * q.y + bjm * qm.y);
             double2 wp = make_double2(0.0, 0.0), Jw = make_double2(0.0, 0.0);
@@ -655,17 +654,16 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
             acc[0] += qb.x * q.x + qb.y * q.y;              // Re<qbar_j|q_j>
             acc[1] += qm.x * qb.x + qm.y * qb.y;            // Re<q_{j-1}|qbar_j>
             acc[2] += qm.x * q.x + qm.y * q.y;              // Re<q_{j-1}|q_j>
-            acc[3] += qm2.x * qb.x + qm2.y * qb.y;          // Re<q_{j-2}|qbar_j>
-            acc[4] += qm2.x * q.x + qm2.y * q.y;            // Re<q_{j-2}|q_j>
-            acc[5] += Jw.x * q.x + Jw.y * q.y;              // Re<J wbar|q_j>
+            acc[3] += qm.x * wp.x + qm.y * wp.y;            // Re<q_{j-1}|wbar'_j>
+            acc[4] += Jw.x * q.x + Jw.y * q.y;              // Re<J wbar|q_j>
         }
     }
     __shared__ double lds[4][64];
     __syncthreads();
-    lds[wave][lane] = acc[5];
+    lds[wave][lane] = acc[4];
     __syncthreads();
     if (wave == 0) a.gpart[(size_t)rtile * Bp + b] += (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
-    adj_store_partials<5>(*reinterpret_cast<const double(*)[5]>(acc), 5, d.part, d.ntile_s, rtile, Bp, b);
+    adj_store_partials<4>(*reinterpret_cast<const double(*)[4]>(acc), 4, d.part, d.ntile_s, rtile, Bp, b);
 }
 
 // qbar_{m-1} = conj(y_{m-1}) psibar' (start of the three-term reverse)
@@ -735,7 +733,7 @@ hipError_t adj_lanczos(const Dev& d, const AdjDev& a, int c, hipStream_t s)
             case 8: hipLaunchKernelGGL((k_adj_spmm_rb<8>), gs, block, 0, s, d, a, c, j); break;
             default: hipLaunchKernelGGL((k_adj_spmm_rb<4>), gs, block, 0, s, d, a, c, j); break;
             }
-            if ((e = launch_reduce(d, 5, d.ntile_s, s)) != hipSuccess) return e;
+            if ((e = launch_reduce_wide(d, 4, d.ntile_s, 0, s)) != hipSuccess) return e;
         }
         return hipGetLastError();
     }

@@ -928,7 +928,21 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     a.pade_stride = (size_t)(16 + max_s + 1) * (dq::kMaxM + 2) * (dq::kMaxM + 2);
     AGET(&a.pade_scratch, sizeof(double2) * a.pade_stride * Bp);
     (void)ad_pbar;
+    // a cycle of the current segment writes its basis, scalars and diagonal
+    // into the tape
+    auto tape_dev = [&](int c) {
+        Dev dc = d;
+        dc.V = a.tapeV + (size_t)c * (m + 1) * nB;
+        dc.sc = a.sc + (size_t)c * (m + 1) * Bp;
+        dc.alpha = a.alpha + (size_t)c * m * Bp;
+        dc.beta = a.beta + (size_t)c * m * Bp;
+        dc.diag = a.tdiag + (size_t)(c / K) * nB;
+        return dc;
+    };
+    const bool obs = d.every > 0 || d.bra != nullptr;
     // ---------------- forward with checkpoints ----------------
+    // the last segment is taped on the way, so the reverse does not recompute it
+    const int64_t last_s0 = (nck - 1) * ckpt;
     for (int64_t step = 0; step < nsteps; ++step) {
         if (step % ckpt == 0) {
             const int64_t k = step / ckpt;
@@ -936,9 +950,18 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
             HIP_TRY(hipMemcpyAsync(ck_de + nB * k, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
             HIP_TRY(hipMemcpyAsync(ck_nrm + (size_t)Bp * k, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
         }
-        HIP_TRY(enqueue_step(sys));
+        if (step < last_s0) {
+            HIP_TRY(enqueue_step(sys));
+            continue;
+        }
+        for (int k = 0; k < K; ++k) {
+            const int c = (int)(step - last_s0) * K + k;
+            HIP_TRY(enqueue_cycle_dev(sys, tape_dev(c), k == 0, k == K - 1, obs));
+            HIP_TRY(hipMemcpyAsync(a.nrm2 + (size_t)c * Bp, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+        }
     }
     HIP_TRY(enqueue_finalize(sys));
+    HIP_TRY(hipMemcpyAsync(a.final_psi, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     sys->step = nsteps;
     DQ_TRY(dq_sync(sys));
@@ -983,27 +1006,23 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     for (int64_t seg = nck - 1; seg >= 0; --seg) {
         const int64_t s0 = seg * ckpt, s1 = std::min<int64_t>(s0 + ckpt, nsteps);
         const int nc = (int)(s1 - s0) * K;
-        HIP_TRY(hipMemcpyAsync(d.psi, ck_psi + nB * seg, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d.de, ck_de + nB * seg, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(d.nrm2, ck_nrm + (size_t)Bp * seg, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
-        int64_t s0v = s0;
-        HIP_TRY(hipMemcpyAsync(d.step, &s0v, sizeof(int64_t), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (int c = 0; c < nc; ++c) {
-            const int k = c % K;
-            Dev dc = d;
-            dc.V = a.tapeV + (size_t)c * (m + 1) * nB;
-            dc.sc = a.sc + (size_t)c * (m + 1) * Bp;
-            dc.alpha = a.alpha + (size_t)c * m * Bp;
-            dc.beta = a.beta + (size_t)c * m * Bp;
-            dc.diag = a.tdiag + (size_t)(c / K) * nB;
-            HIP_TRY(enqueue_cycle_dev(sys, dc, k == 0, k == K - 1, false));
-            HIP_TRY(hipMemcpyAsync(a.nrm2 + (size_t)c * Bp, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
-        }
-        {
-            Dev dc = d;
-            dc.V = a.final_psi;
-            HIP_TRY(dq::launch_prologue(dc, 1, 0, 0, 1, st));
+        if (seg < nck - 1) {             // the last segment was taped by the forward pass
+            HIP_TRY(hipMemcpyAsync(d.psi, ck_psi + nB * seg, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(d.de, ck_de + nB * seg, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+            HIP_TRY(hipMemcpyAsync(d.nrm2, ck_nrm + (size_t)Bp * seg, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+            int64_t s0v = s0;
+            HIP_TRY(hipMemcpyAsync(d.step, &s0v, sizeof(int64_t), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (int c = 0; c < nc; ++c) {
+                const int k = c % K;
+                HIP_TRY(enqueue_cycle_dev(sys, tape_dev(c), k == 0, k == K - 1, false));
+                HIP_TRY(hipMemcpyAsync(a.nrm2 + (size_t)c * Bp, d.nrm2, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
+            }
+            {
+                Dev dc = d;
+                dc.V = a.final_psi;
+                HIP_TRY(dq::launch_prologue(dc, 1, 0, 0, 1, st));
+            }
         }
         a.last_c = nc - 1;
         for (int c = nc - 1; c >= 0; --c) {
