@@ -358,14 +358,18 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
-    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    // eight independent loads in flight per lane, fixed summation order
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double* p = d.part + (size_t)q * ntile * Bp + b;
     int t = wave;
-    for (; t + 48 < ntile; t += 64) {
+    for (; t + 112 < ntile; t += 128) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) a[u] += d.part[((size_t)q * ntile + t + 16 * u) * Bp + b];
+        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(t + 16 * u) * Bp];
     }
-    for (; t < ntile; t += 16) a[0] += d.part[((size_t)q * ntile + t) * Bp + b];
-    lds[wave][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (t + 16 * u < ntile) a[u] += p[(size_t)(t + 16 * u) * Bp];
+    lds[wave][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     __syncthreads();
     if (wave == 0) {
         double s = 0.0;

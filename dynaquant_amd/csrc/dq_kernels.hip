@@ -102,14 +102,18 @@ __device__ __forceinline__ double block_tile_sum(const double* part, int q, int 
         __syncthreads();
         s = (lds[0][0] + lds[1][0]) + (lds[2][0] + lds[3][0]);
     } else {
-        double s2 = 0.0;
+        // eight independent loads in flight per lane, fixed summation order
+        double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double* p = part + (size_t)q * ntile * Bp + b;
         int t = wave;
-        for (; t + 4 < ntile; t += 8) {
-            s += part[((size_t)q * ntile + t) * Bp + b];
-            s2 += part[((size_t)q * ntile + t + 4) * Bp + b];
+        for (; t + 28 < ntile; t += 32) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(t + 4 * u) * Bp];
         }
-        if (t < ntile) s += part[((size_t)q * ntile + t) * Bp + b];
-        s += s2;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t + 4 * u < ntile) a[u] += p[(size_t)(t + 4 * u) * Bp];
+        s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
         lds[wave][lane] = s;
         __syncthreads();
         s = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
