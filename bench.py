@@ -167,7 +167,8 @@ def main():
     # ---- headline: batched mixed precision, hipGraph per step
     s = make_system(dq, t, B, dq.MIXED, W + K + args.profile_steps)
     tuning = s.tuning() if args.no_autotune else s.autotune(reps=2)
-    print("# autotuned launch choices: %s" % tuning, file=sys.stderr, flush=True)
+    print("# launch choices (%s): %s" % ("default" if args.no_autotune else "autotuned", tuning),
+          file=sys.stderr, flush=True)
     s.run(W, graph=True)
     s.run(K, graph=True)
     ms = s.last_run_ms()
