@@ -70,6 +70,11 @@ struct dq_system {
     // host copies
     std::vector<int32_t> rowptr, col;
     std::vector<double> val;
+    // device column (slot) of caller trajectory b; slots are ordered by
+    // n_active, largest first, so the batched kernels can skip rows that are
+    // inactive for a whole slice of slots
+    std::vector<int> slot;
+    std::vector<int32_t> nact_slot;   // n_active per slot (0 for padding)
     // profiling: (class, start, stop) per bracketed launch of the current run
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -268,6 +273,9 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
         std::vector<int32_t> hn(d.Bp, 0);
         std::vector<uint64_t> ht(d.Bp, 0);
         for (int b = 0; b < s->B; ++b) { hn[b] = s->n; ht[b] = (uint64_t)b; }
+        s->slot.resize(s->B);
+        for (int b = 0; b < s->B; ++b) s->slot[b] = b;
+        s->nact_slot = hn;
         hipMemcpyAsync(na, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
         hipMemcpyAsync(ti, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
         d.nact = na; d.traj_id = ti;
@@ -388,17 +396,38 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
 {
     if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
     Dev& d = sys->d;
+    const int B = sys->B;
+    std::vector<int32_t> na(B);
+    for (int b = 0; b < B; ++b) {
+        na[b] = n_active ? n_active[b] : sys->n;
+        if (na[b] < 1 || na[b] > sys->n) return fail(DQ_ERR_INVALID_ARG, "n_active out of range");
+    }
+    // slots ordered by n_active, largest first (stable, so equal sizes keep
+    // the caller's order); the plain path has one slot
+    std::vector<int> order(B);
+    for (int b = 0; b < B; ++b) order[b] = b;
+    if (!d.plain) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return na[x] > na[y]; });
     std::vector<int32_t> hn(d.Bp, 0);
     std::vector<uint64_t> ht(d.Bp, 0);
-    for (int b = 0; b < sys->B; ++b) {
-        ht[b] = traj_ids ? traj_ids[b] : (uint64_t)b;
-        hn[b] = n_active ? n_active[b] : sys->n;
-        if (hn[b] < 1 || hn[b] > sys->n) return fail(DQ_ERR_INVALID_ARG, "n_active out of range");
+    for (int k = 0; k < B; ++k) {
+        const int b = order[k];
+        sys->slot[b] = k;
+        ht[k] = traj_ids ? traj_ids[b] : (uint64_t)b;
+        hn[k] = na[b];
     }
+    sys->nact_slot = hn;
+    sys->drop_graph();
     HIP_TRY(hipMemcpyAsync((void*)d.nact, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipMemcpyAsync((void*)d.traj_id, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
+    // rows a slot does not use must read 0 in every vector; the kernels never
+    // write rows that are inactive for a whole slice, so clear them all here
+    const size_t nB = (size_t)d.n * d.Bp;
+    const size_t vb = d.mixed ? sizeof(float2) : sizeof(double2);
+    HIP_TRY(hipMemsetAsync(d.V, 0, vb * nB * (sys->m + 1), sys->stream));
+    HIP_TRY(hipMemsetAsync(d.psi, 0, sizeof(double2) * nB, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
     sys->have_disorder = false;
+    sys->have_state = false;
     return DQ_OK;
 }
 
@@ -423,13 +452,16 @@ dq_status dq_system_set_state(dq_system* sys, const double* psi)
     Dev& d = sys->d;
     const int n = sys->n, Bp = d.Bp;
     std::vector<double> h((size_t)2 * n * Bp, 0.0);
-    for (int b = 0; b < sys->B; ++b)
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b], na = sys->nact_slot[sb];
         for (int i = 0; i < n; ++i) {
             const double re = psi[((size_t)b * n + i) * 2], im = psi[((size_t)b * n + i) * 2 + 1];
             if (!std::isfinite(re) || !std::isfinite(im)) return fail(DQ_ERR_NONFINITE, "non-finite state");
-            h[((size_t)i * Bp + b) * 2] = re;
-            h[((size_t)i * Bp + b) * 2 + 1] = im;
+            if (i >= na) continue;                 // sites beyond n_active are not part of b
+            h[((size_t)i * Bp + sb) * 2] = re;
+            h[((size_t)i * Bp + sb) * 2 + 1] = im;
         }
+    }
     HIP_TRY(hipMemcpyAsync(d.psi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(dq::launch_set_unit_norm(d, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
@@ -445,11 +477,13 @@ dq_status dq_system_get_state(dq_system* sys, double* psi)
     std::vector<double> h((size_t)2 * n * Bp);
     HIP_TRY(hipStreamSynchronize(sys->stream));
     HIP_TRY(hipMemcpy(h.data(), d.psi, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-    for (int b = 0; b < sys->B; ++b)
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b];
         for (int i = 0; i < n; ++i) {
-            psi[((size_t)b * n + i) * 2] = h[((size_t)i * Bp + b) * 2];
-            psi[((size_t)b * n + i) * 2 + 1] = h[((size_t)i * Bp + b) * 2 + 1];
+            psi[((size_t)b * n + i) * 2] = h[((size_t)i * Bp + sb) * 2];
+            psi[((size_t)b * n + i) * 2 + 1] = h[((size_t)i * Bp + sb) * 2 + 1];
         }
+    }
     return DQ_OK;
 }
 
@@ -697,10 +731,12 @@ dq_status dq_get_populations(dq_system* sys, double* pops, int64_t* nrec_out)
     std::vector<double> h((size_t)nrec * G * d.Bp);
     HIP_TRY(hipStreamSynchronize(sys->stream));
     HIP_TRY(hipMemcpy(h.data(), d.pops, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-    for (int b = 0; b < sys->B; ++b)
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b];
         for (int64_t r = 0; r < nrec; ++r)
             for (int g = 0; g < G; ++g)
-                pops[((size_t)b * nrec + r) * G + g] = h[((size_t)r * G + g) * d.Bp + b];
+                pops[((size_t)b * nrec + r) * G + g] = h[((size_t)r * G + g) * d.Bp + sb];
+    }
     return DQ_OK;
 }
 
@@ -713,11 +749,13 @@ dq_status dq_get_corr(dq_system* sys, double* corr)
     std::vector<double2> h((size_t)(T + 1) * d.Bp);
     HIP_TRY(hipStreamSynchronize(sys->stream));
     HIP_TRY(hipMemcpy(h.data(), d.corr, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
-    for (int b = 0; b < sys->B; ++b)
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b];
         for (int64_t k = 0; k <= T; ++k) {
-            corr[((size_t)b * (T + 1) + k) * 2] = h[(size_t)k * d.Bp + b].x;
-            corr[((size_t)b * (T + 1) + k) * 2 + 1] = h[(size_t)k * d.Bp + b].y;
+            corr[((size_t)b * (T + 1) + k) * 2] = h[(size_t)k * d.Bp + sb].x;
+            corr[((size_t)b * (T + 1) + k) * 2 + 1] = h[(size_t)k * d.Bp + sb].y;
         }
+    }
     return DQ_OK;
 }
 
@@ -727,7 +765,7 @@ dq_status dq_get_error_sum(dq_system* sys, double* err)
     std::vector<double> h(sys->d.Bp);
     HIP_TRY(hipStreamSynchronize(sys->stream));
     HIP_TRY(hipMemcpy(h.data(), sys->d.err_sum, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
-    for (int b = 0; b < sys->B; ++b) err[b] = h[b];
+    for (int b = 0; b < sys->B; ++b) err[b] = h[sys->slot[b]];
     return DQ_OK;
 }
 
@@ -741,12 +779,12 @@ dq_status dq_get_diagonal(dq_system* sys, double* diag)
         std::vector<float> h(nB);
         HIP_TRY(hipMemcpy(h.data(), d.diag, sizeof(float) * nB, hipMemcpyDeviceToHost));
         for (int b = 0; b < sys->B; ++b)
-            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + b];
+            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + sys->slot[b]];
     } else {
         std::vector<double> h(nB);
         HIP_TRY(hipMemcpy(h.data(), d.diag, sizeof(double) * nB, hipMemcpyDeviceToHost));
         for (int b = 0; b < sys->B; ++b)
-            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + b];
+            for (int i = 0; i < sys->n; ++i) diag[(size_t)b * sys->n + i] = h[(size_t)i * d.Bp + sys->slot[b]];
     }
     return DQ_OK;
 }
@@ -759,9 +797,12 @@ dq_status dq_spmv(dq_system* sys, int32_t traj, const double* x, double* y)
     const Dev& d = sys->d;
     const int n = sys->n, Bp = d.Bp;
     const size_t nB = (size_t)n * Bp;
+    const int na = sys->nact_slot[sys->slot[traj]];
+    traj = sys->slot[traj];
+    // rows beyond n_active must stay 0 in every basis slot
     if (d.mixed) {
         std::vector<float2> h(nB, make_float2(0.f, 0.f));
-        for (int i = 0; i < n; ++i) h[(size_t)i * Bp + traj] = make_float2((float)x[2 * i], (float)x[2 * i + 1]);
+        for (int i = 0; i < na; ++i) h[(size_t)i * Bp + traj] = make_float2((float)x[2 * i], (float)x[2 * i + 1]);
         HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(float2) * nB, hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
         HIP_TRY(hipStreamSynchronize(sys->stream));
@@ -769,7 +810,7 @@ dq_status dq_spmv(dq_system* sys, int32_t traj, const double* x, double* y)
         for (int i = 0; i < n; ++i) { y[2 * i] = h[(size_t)i * Bp + traj].x; y[2 * i + 1] = h[(size_t)i * Bp + traj].y; }
     } else {
         std::vector<double2> h(nB, make_double2(0.0, 0.0));
-        for (int i = 0; i < n; ++i) h[(size_t)i * Bp + traj] = make_double2(x[2 * i], x[2 * i + 1]);
+        for (int i = 0; i < na; ++i) h[(size_t)i * Bp + traj] = make_double2(x[2 * i], x[2 * i + 1]);
         HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
         HIP_TRY(hipStreamSynchronize(sys->stream));
@@ -978,24 +1019,25 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
         HIP_TRY(hipMemcpy(corr.data(), d.corr, sizeof(double2) * corr.size(), hipMemcpyDeviceToHost));
     }
     for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b];
         double L = 0.0;
         if (w_pop)
             for (int i = 0; i < n; ++i) {
                 const int g = grp[i];
                 if (g < 0) continue;
-                const double2 p = hpsi[(size_t)i * Bp + b];
+                const double2 p = hpsi[(size_t)i * Bp + sb];
                 L += w_pop[g] * (p.x * p.x + p.y * p.y);
-                hbar[(size_t)i * Bp + b] = make_double2(2.0 * w_pop[g] * p.x, 2.0 * w_pop[g] * p.y);
+                hbar[(size_t)i * Bp + sb] = make_double2(2.0 * w_pop[g] * p.x, 2.0 * w_pop[g] * p.y);
             }
         if (spec)
             for (int64_t k = 0; k <= nsteps; ++k) {
-                const double2 c = corr[(size_t)k * Bp + b];
+                const double2 c = corr[(size_t)k * Bp + sb];
                 const double tr = corr_target ? corr_target[2 * k] : 0.0;
                 const double ti = corr_target ? corr_target[2 * k + 1] : 0.0;
                 const double dr = c.x - tr, di = c.y - ti;
                 const double ws = w_spec * (corr_weight ? corr_weight[k] : 1.0);
                 L += ws * (dr * dr + di * di);
-                cb[(size_t)k * Bp + b] = make_double2(2.0 * ws * dr, 2.0 * ws * di);
+                cb[(size_t)k * Bp + sb] = make_double2(2.0 * ws * dr, 2.0 * ws * di);
             }
         loss[b] = L;
     }
@@ -1040,9 +1082,10 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     HIP_TRY(hipMemcpy(ge.data(), a.grad_e, sizeof(double) * nB, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(gp.data(), a.gpart, sizeof(double) * gp.size(), hipMemcpyDeviceToHost));
     for (int b = 0; b < sys->B; ++b) {
-        for (int i = 0; i < n; ++i) grad_e[(size_t)b * n + i] = ge[(size_t)i * Bp + b];
+        const int sb = sys->slot[b];
+        for (int i = 0; i < n; ++i) grad_e[(size_t)b * n + i] = ge[(size_t)i * Bp + sb];
         double g = 0.0;
-        for (int t = 0; t < d.ntile_s; ++t) g += gp[(size_t)t * Bp + b];
+        for (int t = 0; t < d.ntile_s; ++t) g += gp[(size_t)t * Bp + sb];
         grad_g[b] = g;
     }
     // leave the system at the final state of the forward run

@@ -146,7 +146,10 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
     double acc[TPL][2];
 #pragma unroll
     for (int t = 0; t < TPL; ++t) { acc[t][0] = 0.0; acc[t][1] = 0.0; }
-    if (rb < nblk) {
+    // slots are sorted by n_active: a row block at or beyond the slice's first
+    // n_active is inactive for every lane (its w rows stay 0)
+    const int slice_nact = uniform(d.nact[ttile * TW]);
+    if (rb < nblk && rb * kRB < slice_nact) {
         const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
         VT ar[TPL][kRB], ai[TPL][kRB];
 #pragma unroll
@@ -434,7 +437,9 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     double acc[2 * KW + 1];
 #pragma unroll
     for (int q = 0; q < 2 * KW + 1; ++q) acc[q] = 0.0;
-    const int r0 = rtile * d.rt_e, r1 = min(d.n, r0 + d.rt_e);
+    // rows at or beyond the first (largest) n_active of the 16 sorted slots
+    // are inactive for all of them: skipped, they stay 0
+    const int r0 = rtile * d.rt_e, r1 = min(min(d.n, r0 + d.rt_e), (int)uniform(d.nact[ttile * 16]));
     // software pipeline: the vectors of row i+4 load while row i is combined
     V2 nk[KW], nw;
     if (r0 + wave < r1) {
@@ -569,7 +574,9 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
     }
     const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
     double acc = 0.0;
-    const int r0 = rtile * d.rt_e, r1 = min(d.n, r0 + d.rt_e);
+    // rows at or beyond the first (largest) n_active of the 16 sorted slots
+    // are inactive for all of them: skipped, they stay 0
+    const int r0 = rtile * d.rt_e, r1 = min(min(d.n, r0 + d.rt_e), (int)uniform(d.nact[ttile * 16]));
     V2 nk[KW];
     double2 np = make_double2(0.0, 0.0);
     if (r0 + wave < r1) {

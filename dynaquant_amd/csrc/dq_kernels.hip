@@ -219,7 +219,10 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
     for (int q = 0; q < kMaxGroups + 2; ++q) acc[q] = 0.0;
     V2* v0 = reinterpret_cast<V2*>(d.V);
     VT* diag = reinterpret_cast<VT*>(d.diag);
-    for (int i = t.rfirst; i < t.r1; i += t.rstep) {
+    // trajectory slots are sorted by n_active (largest first), so rows at or
+    // beyond the slice's first n_active are inactive for all of its lanes
+    const int rend = PLAIN ? t.r1 : min(t.r1, (int)uniform(d.nact[b & ~63]));
+    for (int i = t.rfirst; i < rend; i += t.rstep) {
         const size_t e = (size_t)i * d.Bp + b;
         double2 p = d.psi[e];
         if (renorm) {

@@ -86,14 +86,18 @@ dq_status dq_system_set_hamiltonian_csr(dq_system *sys, int32_t n, const int32_t
 dq_status dq_system_set_ou_params(dq_system *sys, const double *tau_fs, const double *sigma);
 /* Philox key word 1 of each trajectory (NULL: 0..B-1) and its active site
  * count (NULL: all N).  A trajectory with n_active < N propagates under the
- * leading n_active x n_active block of H.  Clears observables. */
+ * leading n_active x n_active block of H.  Clears observables and the state
+ * (call dq_system_reset_disorder and dq_system_set_state again).  Internally
+ * the trajectories are ordered by n_active so that rows no trajectory of a
+ * slice uses are skipped; every output stays in the caller's order. */
 dq_status dq_system_set_trajectories(dq_system *sys, const uint64_t *traj_ids,
                                      const int32_t *n_active);
 /* Draw static disorder and the stationary OU start for every trajectory,
  * reset the step counter and the error sums (systemClass.cu:759-804,
  * 1028-1036, 1552-1608). */
 dq_status dq_system_reset_disorder(dq_system *sys);
-/* psi[b][i] interleaved complex, b < B, i < N (systemClass.cu:1325-1508) */
+/* psi[b][i] interleaved complex, b < B, i < N; entries at i >= n_active[b]
+ * are ignored (systemClass.cu:1325-1508) */
 dq_status dq_system_set_state(dq_system *sys, const double *psi);
 dq_status dq_system_get_state(dq_system *sys, double *psi);
 /* Observables: site i adds |psi_i|^2 to group[i] (< 0: none), recorded
