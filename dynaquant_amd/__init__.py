@@ -53,11 +53,12 @@ class Tuning(C.Structure):
     _fields_ = [
         ("spmm_rows_per_wave", C.c_int32), ("spmm_traj_per_lane", C.c_int32),
         ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("spmm_pipeline", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("concurrent_slices", C.c_int32), ("reserved", C.c_int32 * 2),
     ]
 
 
-TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows", "spmm_pipeline"]
+TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows", "spmm_pipeline",
+               "concurrent_slices"]
 
 
 class TubeSpec(C.Structure):

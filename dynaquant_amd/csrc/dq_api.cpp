@@ -59,7 +59,11 @@ struct dq_system {
     dq_config cfg{};
     Dev d{};
     hipStream_t stream = nullptr;
+    static constexpr int kMaxSlices = 4;
+    hipStream_t side[kMaxSlices - 1] = {};   // streams of slices 1.. in captured steps
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxSlices - 1] = {};
+    int slices = 1;                   // captured steps run this many trajectory slices concurrently
     hipGraphExec_t graph = nullptr;
     double last_ms = 0.0;
     int n = 0, B = 0, Bp = 0, m = 0, K = 1;
@@ -94,6 +98,11 @@ struct dq_system {
         for (void* p : allocs) hipFree(p);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        for (int k = 0; k < kMaxSlices - 1; ++k) {
+            if (ev_join[k]) hipEventDestroy(ev_join[k]);
+            if (side[k]) hipStreamDestroy(side[k]);
+        }
         if (stream) hipStreamDestroy(stream);
     }
 
@@ -212,6 +221,15 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     if (e != hipSuccess) { delete s; return fail(DQ_ERR_HIP, std::string("stream: ") + hipGetErrorString(e)); }
     hipEventCreate(&s->ev0);
     hipEventCreate(&s->ev1);
+    e = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming);
+    for (int k = 0; k < dq_system::kMaxSlices - 1 && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&s->side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_join[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        delete s;
+        return fail(DQ_ERR_HIP, std::string("stream/event: ") + hipGetErrorString(e));
+    }
     Dev& d = s->d;
     d.n = s->n; d.Bp = s->Bp; d.B = s->B; d.m = s->m;
     d.mixed = cfg->precision == DQ_PREC_MIXED;
@@ -221,6 +239,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.spmm_tpl = (d.mixed && d.Bp % 128 == 0) ? 2 : 1;
     d.spmm_phases = 1;
     d.spmm_pipe = 0;
+    d.s0 = 0;
+    d.sn = d.Bp;
     d.rt_s = plain ? 64 : 4 * d.rb;      // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
@@ -584,9 +604,8 @@ static hipError_t collect_timings(dq_system* sys)
 extern "C" {
 
 static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
-                                    bool obs)
+                                    bool obs, hipStream_t st, bool advance)
 {
-    hipStream_t st = sys->stream;
     hipError_t e;
     if ((e = timed(sys, DQ_K_PROLOGUE, [&] { return dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st); })) != hipSuccess) return e;
     if (first_in_step && obs && (e = timed(sys, DQ_K_NORM, [&] { return dq::launch_reduce_obs(d, st); })) != hipSuccess) return e;
@@ -599,7 +618,13 @@ static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_
     if ((e = timed(sys, DQ_K_SPMM_LAST, [&] { return dq::launch_spmm(d, d.m, 1, st); })) != hipSuccess) return e;
     if ((e = timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); })) != hipSuccess) return e;
     if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_combine(d, st); })) != hipSuccess) return e;
-    return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, last_in_step ? 1 : 0, st); });
+    return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, (last_in_step && advance) ? 1 : 0, st); });
+}
+
+static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
+                                    bool obs)
+{
+    return enqueue_cycle_dev(sys, d, first_in_step, last_in_step, obs, sys->stream, true);
 }
 
 static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
@@ -614,6 +639,35 @@ static hipError_t enqueue_step(dq_system* sys)
     for (int k = 0; k < sys->K; ++k)
         if ((e = enqueue_cycle(sys, k == 0, k == sys->K - 1)) != hipSuccess) return e;
     return e;
+}
+
+// One step with the trajectory slots split into `slices` equal slices, each on
+// its own stream (fork/join by events; used inside graph capture, where the
+// chains of small latency-bound kernels fill each other's gaps).  The
+// per-trajectory arithmetic is unchanged, so results are bitwise those of one
+// stream.  The step counter moves once, after all slices.
+static hipError_t enqueue_step_slices(dq_system* sys)
+{
+    const Dev& d = sys->d;
+    const bool obs = d.every > 0 || d.bra != nullptr;
+    const int ns = sys->slices;
+    hipError_t e;
+    if ((e = hipEventRecord(sys->ev_fork, sys->stream)) != hipSuccess) return e;
+    for (int h = 1; h < ns; ++h)
+        if ((e = hipStreamWaitEvent(sys->side[h - 1], sys->ev_fork, 0)) != hipSuccess) return e;
+    for (int h = 0; h < ns; ++h) {
+        Dev dh = d;
+        dh.sn = d.Bp / ns;
+        dh.s0 = h * dh.sn;
+        hipStream_t st = h ? sys->side[h - 1] : sys->stream;
+        for (int k = 0; k < sys->K; ++k)
+            if ((e = enqueue_cycle_dev(sys, dh, k == 0, k == sys->K - 1, obs, st, false)) != hipSuccess) return e;
+    }
+    for (int h = 1; h < ns; ++h) {
+        if ((e = hipEventRecord(sys->ev_join[h - 1], sys->side[h - 1])) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(sys->stream, sys->ev_join[h - 1], 0)) != hipSuccess) return e;
+    }
+    return dq::launch_advance_step(d, sys->stream);
 }
 
 static hipError_t enqueue_finalize(dq_system* sys)
@@ -656,10 +710,11 @@ dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
     if ((sys->every > 0 || sys->have_bra) && sys->step + nsteps > sys->max_steps)
         return fail(DQ_ERR_INVALID_ARG, "run exceeds max_steps of the observables");
     hipStream_t st = sys->stream;
+    if (sys->profiling) use_graph = 0;
     if (use_graph && nsteps > 0 && !sys->graph) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t e = enqueue_step(sys);
+        hipError_t e = sys->slices > 1 ? enqueue_step_slices(sys) : enqueue_step(sys);
         hipError_t e2 = hipStreamEndCapture(st, &g);
         if (e != hipSuccess || e2 != hipSuccess)
             return fail(DQ_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
@@ -667,7 +722,6 @@ dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
         hipGraphDestroy(g);
         if (e != hipSuccess) { sys->graph = nullptr; return fail(DQ_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
     }
-    if (sys->profiling) use_graph = 0;
     HIP_TRY(hipEventRecord(sys->ev0, st));
     for (int64_t k = 0; k < nsteps; ++k) {
         if (use_graph) HIP_TRY(hipGraphLaunch(sys->graph, st));
@@ -1129,6 +1183,15 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "spmm_pipeline must be 0 or 1";
         return false;
     }
+    const int ns = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
+    if (ns != 1 && ns != 2 && ns != 4) {
+        *why = "concurrent_slices must be 1, 2 or 4";
+        return false;
+    }
+    if (d.Bp % (64 * ns) != 0) {
+        *why = "concurrent_slices needs slices of a multiple of 64 trajectory slots";
+        return false;
+    }
     return true;
 }
 
@@ -1140,6 +1203,7 @@ static void get_tuning(const dq_system* sys, dq_tuning* t)
     t->spmm_phases = sys->d.spmm_phases;
     t->tile_rows = sys->d.plain ? 0 : sys->d.rt_e;
     t->spmm_pipeline = sys->d.spmm_pipe;
+    t->concurrent_slices = sys->slices;
 }
 
 static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
@@ -1157,6 +1221,7 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     d.spmm_tpl = t.spmm_traj_per_lane;
     d.spmm_phases = t.spmm_phases;
     d.spmm_pipe = t.spmm_pipeline;
+    sys->slices = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
     d.rt_e = t.tile_rows;
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     return DQ_OK;
@@ -1165,10 +1230,8 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
 // one Lanczos cycle without the kernels that touch the propagated state
 // (noise, Pade error sum, renormalisation): SpMMs, scalars, updates, reductions
 // and the combine (whose psi output the caller restores)
-static hipError_t enqueue_tuning_cycle(dq_system* sys)
+static hipError_t enqueue_tuning_cycle(dq_system* sys, const Dev& d, hipStream_t st)
 {
-    const Dev& d = sys->d;
-    hipStream_t st = sys->stream;
     hipError_t e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
@@ -1178,6 +1241,38 @@ static hipError_t enqueue_tuning_cycle(dq_system* sys)
     }
     if ((e = dq::launch_spmm(d, d.m, 1, st)) != hipSuccess) return e;
     return dq::launch_combine(d, st);
+}
+
+static hipError_t enqueue_tuning_cycle(dq_system* sys)
+{
+    return enqueue_tuning_cycle(sys, sys->d, sys->stream);
+}
+
+// the tuning cycle over `ns` concurrent slices, captured as one graph
+static hipError_t capture_tuning_slices(dq_system* sys, int ns, hipGraphExec_t* out)
+{
+    const Dev& d = sys->d;
+    hipGraph_t g;
+    hipError_t e = hipStreamBeginCapture(sys->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(sys->ev_fork, sys->stream);
+    for (int h = 1; h < ns && e == hipSuccess; ++h) e = hipStreamWaitEvent(sys->side[h - 1], sys->ev_fork, 0);
+    for (int h = 0; h < ns && e == hipSuccess; ++h) {
+        Dev dh = d;
+        dh.sn = d.Bp / ns;
+        dh.s0 = h * dh.sn;
+        e = enqueue_tuning_cycle(sys, dh, h ? sys->side[h - 1] : sys->stream);
+    }
+    for (int h = 1; h < ns && e == hipSuccess; ++h) {
+        e = hipEventRecord(sys->ev_join[h - 1], sys->side[h - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(sys->stream, sys->ev_join[h - 1], 0);
+    }
+    hipError_t e2 = hipStreamEndCapture(sys->stream, &g);
+    if (e != hipSuccess) return e;
+    if (e2 != hipSuccess) return e2;
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    return e;
 }
 
 static std::string tuning_key(const dq_system* sys)
@@ -1236,8 +1331,9 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 dq_tuning t;
                 std::memset(&t, 0, sizeof(t));
                 std::string why;
-                if (std::sscanf(line + key.size(), "%d %d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
-                                &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline) == 5 && tuning_valid(sys, t, &why)) {
+                if (std::sscanf(line + key.size(), "%d %d %d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
+                                &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline, &t.concurrent_slices) == 6 &&
+                    tuning_valid(sys, t, &why)) {
                     best = t;
                     hit = true;
                 }
@@ -1291,6 +1387,34 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
             }
         }
     }
+    // concurrent slices: the chosen kernels, captured as graphs of 1, 2, 4 slices
+    if (e == hipSuccess && st == DQ_OK) {
+        if ((st = apply_tuning(sys, best)) == DQ_OK) {
+            double best_ms = 1e300;
+            for (int ns : {1, 2, 4}) {
+                dq_tuning t = best;
+                t.concurrent_slices = ns;
+                std::string why;
+                if (!tuning_valid(sys, t, &why)) continue;
+                hipGraphExec_t ge = nullptr;
+                if ((e = capture_tuning_slices(sys, ns, &ge)) != hipSuccess) break;
+                float ms_min = 1e30f;
+                e = hipGraphLaunch(ge, sys->stream);                            // warm-up
+                for (int r = 0; r < reps && e == hipSuccess; ++r) {
+                    float ms = 0.f;
+                    if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
+                    if ((e = hipGraphLaunch(ge, sys->stream)) != hipSuccess) break;
+                    if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
+                    if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
+                    if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
+                    ms_min = std::min(ms_min, ms);
+                }
+                hipGraphExecDestroy(ge);
+                if (e != hipSuccess) break;
+                if (ms_min < best_ms) { best_ms = ms_min; best.concurrent_slices = ns; }
+            }
+        }
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(d.psi, saved, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
     hipFree(saved);
@@ -1299,8 +1423,8 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     DQ_TRY(apply_tuning(sys, best));
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
-                         best.spmm_phases, best.tile_rows, best.spmm_pipeline);
+            std::fprintf(f, "%s %d %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
+                         best.spmm_phases, best.tile_rows, best.spmm_pipeline, best.concurrent_slices);
             std::fclose(f);
         }
     }

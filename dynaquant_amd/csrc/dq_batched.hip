@@ -252,13 +252,14 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
 }
 
 template <int RB, bool PIPE>
-static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, int ntt, hipStream_t s)
+static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, int first, int ntt, hipStream_t s)
 {
     const dim3 grid(bpad8(d.ntile_s * ntt)), block(kThreads);
     for (int ph = 0; ph < phases; ++ph) {
-        if (tpl == 2) hipLaunchKernelGGL((k_spmm_b<float, 2, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
-        else if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float, 1, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
-        else hipLaunchKernelGGL((k_spmm_b<double, 1, RB, PIPE>), grid, block, 0, s, d, j, last, ph * ntt, ntt);
+        const int tt0 = first + ph * ntt;
+        if (tpl == 2) hipLaunchKernelGGL((k_spmm_b<float, 2, RB, PIPE>), grid, block, 0, s, d, j, last, tt0, ntt);
+        else if (d.mixed) hipLaunchKernelGGL((k_spmm_b<float, 1, RB, PIPE>), grid, block, 0, s, d, j, last, tt0, ntt);
+        else hipLaunchKernelGGL((k_spmm_b<double, 1, RB, PIPE>), grid, block, 0, s, d, j, last, tt0, ntt);
     }
 }
 
@@ -266,16 +267,16 @@ hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
     // trajectory slices of 64 * TPL; the slices run as `phases` launches of
     // ntt slices each, so that one launch's gathered rows fit the L2 of an XCD
-    const int tpl = (d.mixed && d.spmm_tpl == 2 && d.Bp % 128 == 0) ? 2 : 1;
-    const int nslice = d.Bp / (64 * tpl);
+    const int tpl = (d.mixed && d.spmm_tpl == 2 && d.sn % 128 == 0) ? 2 : 1;
+    const int nslice = d.sn / (64 * tpl), first = d.s0 / (64 * tpl);
     int phases = d.spmm_phases < 1 ? 1 : d.spmm_phases;
     while (nslice % phases) --phases;
     const int ntt = nslice / phases;
     const bool pipe = d.spmm_pipe != 0;
     switch (d.rb) {
-    case 2: pipe ? launch_spmm_rb<2, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<2, false>(d, j, last, tpl, phases, ntt, s); break;
-    case 8: pipe ? launch_spmm_rb<8, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<8, false>(d, j, last, tpl, phases, ntt, s); break;
-    default: pipe ? launch_spmm_rb<4, true>(d, j, last, tpl, phases, ntt, s) : launch_spmm_rb<4, false>(d, j, last, tpl, phases, ntt, s); break;
+    case 2: pipe ? launch_spmm_rb<2, true>(d, j, last, tpl, phases, first, ntt, s) : launch_spmm_rb<2, false>(d, j, last, tpl, phases, first, ntt, s); break;
+    case 8: pipe ? launch_spmm_rb<8, true>(d, j, last, tpl, phases, first, ntt, s) : launch_spmm_rb<8, false>(d, j, last, tpl, phases, first, ntt, s); break;
+    default: pipe ? launch_spmm_rb<4, true>(d, j, last, tpl, phases, first, ntt, s) : launch_spmm_rb<4, false>(d, j, last, tpl, phases, first, ntt, s); break;
     }
     return hipGetLastError();
 }
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     __shared__ double s_sj[64];
     __shared__ double2 s_G[kMaxM + 1][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.x * 64 + lane;
+    const int b = d.s0 + blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
     // <v_j|w> reduced over the SpMM tiles by k_reduce_wide into red[kRedC..]
     const double cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
@@ -359,7 +360,7 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
     __shared__ double lds[16][64];
     const int q = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.x * 64 + lane;
+    const int b = d.s0 + blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
     // eight independent loads in flight per lane, fixed summation order
     double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -383,7 +384,7 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
 
 hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_wide, dim3(d.Bp / 64, nq), dim3(1024), 0, s, d, ntile, dst);
+    hipLaunchKernelGGL(k_reduce_wide, dim3(d.sn / 64, nq), dim3(1024), 0, s, d, ntile, dst);
     return hipGetLastError();
 }
 
@@ -392,7 +393,7 @@ hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
     hipError_t e = launch_reduce_wide(d, 2, d.ntile_s, kRedC(d.m), s);
     if (e != hipSuccess) return e;
     const int ny = j > 0 ? (j + 3) / 4 : 1;
-    hipLaunchKernelGGL(k_scalars_hb, dim3(d.Bp / 64, ny), dim3(kThreads), 0, s, d, j);
+    hipLaunchKernelGGL(k_scalars_hb, dim3(d.sn / 64, ny), dim3(kThreads), 0, s, d, j);
     return hipGetLastError();
 }
 
@@ -408,10 +409,10 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     __shared__ double2 s_c[4 * KW][16];
     __shared__ double s_w[16];
     __shared__ double lds[4][2 * KW + 1][64];
-    const int ntt = d.Bp / 16;
+    const int ntt = d.sn / 16;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_e * ntt) return;
-    const int rtile = logical / ntt, ttile = logical % ntt;
+    const int rtile = logical / ntt, ttile = d.s0 / 16 + logical % ntt;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = lane & 15, kg = lane >> 4;
     const int b = ttile * 16 + t;
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
 
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s)
 {
-    dim3 grid(bpad8(d.ntile_e * (d.Bp / 16))), block(kThreads);
+    dim3 grid(bpad8(d.ntile_e * (d.sn / 16))), block(kThreads);
     const int nv = j + 1;
     if (d.mixed) {
         if (nv <= 8) hipLaunchKernelGGL((k_update_b<float, 2>), grid, block, 0, s, d, j);
@@ -552,10 +553,10 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
     using V2 = typename BVec2<VT>::T;
     __shared__ double2 s_y[4 * KW][16];
     __shared__ double lds[4][64];
-    const int ntt = d.Bp / 16;
+    const int ntt = d.sn / 16;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_e * ntt) return;
-    const int rtile = logical / ntt, ttile = logical % ntt;
+    const int rtile = logical / ntt, ttile = d.s0 / 16 + logical % ntt;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int t = lane & 15, kg = lane >> 4;
     const int b = ttile * 16 + t;
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 
 hipError_t launch_combine_b(const Dev& d, hipStream_t s)
 {
-    dim3 grid(bpad8(d.ntile_e * (d.Bp / 16))), block(kThreads);
+    dim3 grid(bpad8(d.ntile_e * (d.sn / 16))), block(kThreads);
     if (d.mixed) {
         if (d.m <= 8) hipLaunchKernelGGL((k_combine_b<float, 2>), grid, block, 0, s, d);
         else if (d.m <= 16) hipLaunchKernelGGL((k_combine_b<float, 4>), grid, block, 0, s, d);

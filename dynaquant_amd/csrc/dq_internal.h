@@ -27,6 +27,7 @@ struct Dev {
     int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
     int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
     int spmm_pipe;                  // batched SpMM: 1 = two gather buffers in flight
+    int s0, sn;                     // batched kernels work on trajectory slots [s0, s0 + sn)
     int S;                          // OU substeps
     uint64_t seed;
     double subdt, sqrt_subdt;

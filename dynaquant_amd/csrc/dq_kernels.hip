@@ -132,12 +132,12 @@ template <bool PLAIN>
 __device__ __forceinline__ Tile elem_tile(const Dev& d)
 {
     Tile t;
-    const int ntt = PLAIN ? 1 : d.Bp / 64;
+    const int ntt = PLAIN ? 1 : d.sn / 64;
     const int nblk = d.ntile_e * ntt;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     t.ok = logical < nblk;
     t.rtile = logical / ntt;
-    const int ttile = logical % ntt;
+    const int ttile = (PLAIN ? 0 : d.s0 / 64) + logical % ntt;
     const int tid = threadIdx.x;
     t.b = PLAIN ? 0 : ttile * 64 + (tid & 63);
     t.r0 = t.rtile * d.rt_e;
@@ -147,7 +147,7 @@ __device__ __forceinline__ Tile elem_tile(const Dev& d)
     return t;
 }
 
-static inline int elem_grid(const Dev& d) { return pad8(d.ntile_e * (d.plain ? 1 : d.Bp / 64)); }
+static inline int elem_grid(const Dev& d) { return pad8(d.ntile_e * (d.plain ? 1 : d.sn / 64)); }
 
 // ---------------------------------------------------------------------------
 // Static disorder and stationary OU start (systemClass.cu:759-804, 1028-1036)
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
     const bool corr_on = d.bra != nullptr && step <= d.max_steps;
     const bool grp = rec_idx >= 0 && !d.site_mode;
     if (!grp && !corr_on) return;
-    const int b = PLAIN ? 0 : blockIdx.x * 64 + (threadIdx.x & 63);
+    const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     if (grp) {
         for (int g = 0; g < d.ngroups; ++g) {
             double s = block_tile_sum<PLAIN>(d.part, g, d.ntile_e, d.Bp, b);
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
 hipError_t launch_reduce_obs(const Dev& d, hipStream_t s)
 {
     if (d.plain) hipLaunchKernelGGL((k_reduce_obs<true>), dim3(1), dim3(kThreads), 0, s, d);
-    else hipLaunchKernelGGL((k_reduce_obs<false>), dim3(d.Bp / 64), dim3(kThreads), 0, s, d);
+    else hipLaunchKernelGGL((k_reduce_obs<false>), dim3(d.sn / 64), dim3(kThreads), 0, s, d);
     return hipGetLastError();
 }
 
@@ -561,7 +561,7 @@ template <bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_reduce(Dev d, int ntile)
 {
     const int q = blockIdx.y;
-    const int b = PLAIN ? 0 : blockIdx.x * 64 + (threadIdx.x & 63);
+    const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     double s = block_tile_sum<PLAIN>(d.part, q, ntile, d.Bp, b);
     if (threadIdx.x < (PLAIN ? 1u : 64u)) d.red[(size_t)q * d.Bp + b] = s;
 }
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(Dev d, int ntile)
 hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s)
 {
     if (d.plain) hipLaunchKernelGGL((k_reduce<true>), dim3(1, nq), dim3(kThreads), 0, s, d, ntile);
-    else hipLaunchKernelGGL((k_reduce<false>), dim3(d.Bp / 64, nq), dim3(kThreads), 0, s, d, ntile);
+    else hipLaunchKernelGGL((k_reduce<false>), dim3(d.sn / 64, nq), dim3(kThreads), 0, s, d, ntile);
     return hipGetLastError();
 }
 
@@ -766,7 +766,7 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
     extern __shared__ double sm[];
     __shared__ double al[kMaxM], be[kMaxM];
     __shared__ double s_hq;
-    const int b = blockIdx.x;
+    const int b = d.s0 + blockIdx.x;
     const int m = d.m, Bp = d.Bp;
     // |H v_m|^2 summed over the last SpMM's tiles
     double hq2 = 0.0;
@@ -834,7 +834,7 @@ static size_t pade_lds_bytes() { return sizeof(double) * 6 * kNN; }
 
 hipError_t launch_pade(const Dev& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pade, dim3(d.Bp), dim3(kThreads), pade_lds_bytes(), s, d);
+    hipLaunchKernelGGL(k_pade, dim3(d.sn), dim3(kThreads), pade_lds_bytes(), s, d);
     return hipGetLastError();
 }
 
@@ -936,7 +936,7 @@ hipError_t launch_combine(const Dev& d, hipStream_t s)
 template <bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_finish_norm(Dev d, int advance_step)
 {
-    const int b = PLAIN ? 0 : blockIdx.x * 64 + (threadIdx.x & 63);
+    const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     const double s = block_tile_sum<PLAIN>(d.part, 0, d.ntile_e, d.Bp, b);
     if (threadIdx.x < (PLAIN ? 1u : 64u)) d.nrm2[b] = s;
     if (advance_step && blockIdx.x == 0 && threadIdx.x == 0) *d.step = *d.step + 1;
@@ -945,7 +945,7 @@ __global__ __launch_bounds__(kThreads) void k_finish_norm(Dev d, int advance_ste
 hipError_t launch_finish_norm(const Dev& d, int advance_step, hipStream_t s)
 {
     if (d.plain) hipLaunchKernelGGL((k_finish_norm<true>), dim3(1), dim3(kThreads), 0, s, d, advance_step);
-    else hipLaunchKernelGGL((k_finish_norm<false>), dim3(d.Bp / 64), dim3(kThreads), 0, s, d, advance_step);
+    else hipLaunchKernelGGL((k_finish_norm<false>), dim3(d.sn / 64), dim3(kThreads), 0, s, d, advance_step);
     return hipGetLastError();
 }
 

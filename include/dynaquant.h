@@ -227,7 +227,10 @@ typedef struct dq_tuning {
                                     and noise kernels */
     int32_t spmm_pipeline;       /* 0 or 1: gathers of the next 8 union columns in flight while
                                     the current 8 are multiplied in */
-    int32_t reserved[3];
+    int32_t concurrent_slices;   /* 1, 2 or 4: a graph-captured step runs that many equal slices
+                                    of the trajectory slots on their own streams (each slice a
+                                    multiple of 64 slots) */
+    int32_t reserved[2];
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

@@ -74,7 +74,7 @@ def test_invalid_tuning_rejected(engine):
     sysd = ring_tube(engine)
     s = make_engine(engine, sysd, 16, 1.0, B=64, force_batched=True, precision=engine.FP64)
     for bad in (dict(spmm_rows_per_wave=3), dict(spmm_traj_per_lane=2), dict(spmm_phases=3),
-                dict(tile_rows=100)):
+                dict(tile_rows=100), dict(concurrent_slices=3), dict(concurrent_slices=2)):
         with pytest.raises(engine.DQError):
             s.set_tuning(**bad)
     s.close()
@@ -109,3 +109,30 @@ def test_autotune_keeps_state_and_caches(engine, tmp_path):
         s.close()
     # the tuned system continues the same trajectories (tile_rows may regroup FP64 sums)
     assert np.abs(runs[0][0] - runs[1][0]).max() < 1e-6
+
+
+def test_concurrent_slices_bitwise_equal(engine):
+    """a captured step over 2 or 4 concurrent trajectory slices computes every
+    trajectory exactly as one stream does (and as the eager path does)"""
+    sysd = ring_tube(engine)
+    ref = _run(engine, sysd, engine.MIXED, 256, dict(concurrent_slices=1))
+    for ns in (2, 4):
+        pops, psi, tun = _run(engine, sysd, engine.MIXED, 256, dict(concurrent_slices=ns))
+        assert tun["concurrent_slices"] == ns
+        assert np.array_equal(pops, ref[0]) and np.array_equal(psi, ref[1]), ns
+    # FP64 with ragged trajectories: slices of sorted slots, against the oracle-held eager path
+    n = sysd["n"]
+    na = np.array([600, 120, 480, 360] * 32, np.int32)
+    psi0 = np.zeros((128, n), complex)
+    psi0[:, 0] = 1.0
+    out = []
+    for ns, graph in ((1, False), (2, True), (2, False)):
+        s = make_engine(engine, sysd, 12, 1.0, B=128, force_batched=True, n_active=na)
+        s.set_tuning(concurrent_slices=ns)
+        s.set_state(psi0)
+        s.set_observables((np.arange(n) // 150).astype(np.int32), 4, 5, 20)
+        s.run(20, graph=graph)
+        out.append((s.populations(), s.get_state()))
+        s.close()
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
