@@ -569,7 +569,7 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
             corr[2 * step + 1] = ci;
         }
         if (step == nsteps) break;
-        orc_ou_step(p, sp, traj, step, de);
+        orc_noise_step(p, sp, traj, step, de);
         for (int i = 0; i < n; ++i) diag[i] = de[i] + e0[i];
         for (int k = 0; k < K; ++k) {
             if (cycle_forward(&cyc, diag, psi, tol) != 0) { loss = NAN; goto out; }
@@ -616,7 +616,7 @@ double orc_adjoint(const orc_params *p, const orc_site_params *sp,
         memcpy(de, ck_de + (size_t)n * seg, n * sizeof(double));
         memcpy(seg_psi, psi, cs * sizeof(double));
         for (int64_t step = s0; step < s1; ++step) {
-            orc_ou_step(p, sp, traj, step, de);
+            orc_noise_step(p, sp, traj, step, de);
             double *dg = seg_diag + (size_t)n * (step - s0);
             for (int i = 0; i < n; ++i) dg[i] = de[i] + e0[i];
             for (int k = 0; k < K; ++k) {

@@ -114,11 +114,22 @@ typedef struct {
     uint64_t seed;
 } orc_params;
 
+/* site-energy noise models (systemClass.cu:1881-2013): Kubo-Anderson /
+ * Ornstein-Uhlenbeck (SIM_TYPE_KUBO_ANDERSON), Haken-Strobl-Reineker white
+ * noise (SIM_TYPE_HSR), Zhong-Zhao real noise (SIM_TYPE_ZZ_REAL_NOISE), none
+ * (SIM_TYPE_CONST_HAM) */
+enum { ORC_NOISE_OU = 0, ORC_NOISE_HSR = 1, ORC_NOISE_ZZ = 2, ORC_NOISE_NONE = 3 };
+
 typedef struct {
     const double *ou_tau;    /* [n] fs */
     const double *ou_sigma;  /* [n] cm^-1 */
     const double *e_site;    /* [n] site energies before disorder */
     const double *e_shift;   /* [n] or NULL: added to e0 (adjoint direction) */
+    int noise_model;         /* ORC_NOISE_* */
+    int zz_nosc;             /* ZZ oscillators per site */
+    const double *hsr_sigma; /* [n] cm^-1, HSR standard deviations */
+    const double *zz_w2pi;   /* [zz_nosc] omega / (2 pi) in 1/fs */
+    const double *zz_coeff;  /* [zz_nosc] amplitudes in cm^-1 */
 } orc_site_params;
 
 /* Run one trajectory `traj`.  Outputs (any may be NULL):
@@ -148,6 +159,24 @@ void orc_init_disorder(const orc_params *p, const orc_site_params *sp,
 /* One OU update of step `step` (gpuSysHelper.cu:73-128). */
 void orc_ou_step(const orc_params *p, const orc_site_params *sp,
                  uint64_t traj, int64_t step, double *de);
+/* The noise offset of step `step` for the selected model: OU update of de
+ * (above); HSR de_i = sigma_i Z(i, step, stream 4) (gpuSysHelper.cu:32-66);
+ * ZZ de_i = sum_k c_k cos(2 pi (u_ik + w_k t)), t = step dt, u_ik uniform
+ * phases of Philox stream 5 fixed per trajectory (gpuSysHelper.cu:134-177,
+ * systemClass.cu:1075-1243); none: 0. */
+void orc_noise_step(const orc_params *p, const orc_site_params *sp,
+                    uint64_t traj, int64_t step, double *de);
+/* uniform phase u in [0,1) of ZZ oscillator k of `site` (Philox stream 5) */
+double orc_zz_phase(uint64_t seed, uint64_t traj, uint64_t site, uint64_t k);
+/* cos(2 pi (u + w t)) with the argument reduced to [0,1) and the oracle's
+ * own sin/cos of 2 pi x */
+double orc_zz_cos(double u, double w2pi, double t);
+/* ZZ amplitudes from a spectral density J(omega) sampled every domega
+ * (systemClass.cu:1118-1145): G = J / (1 - exp(-omega/(kB T))),
+ * c = sqrt(2) sqrt(G domega / pi), kB = 0.695035 cm^-1/K; omega/(2 pi) in
+ * 1/fs with the reference's conversion 5308.84 cm^-1 fs (includeFiles.cu:16). */
+void orc_zz_oscillators(int nosc, const double *omega_cm, const double *J, double domega,
+                        double temperature, double *w2pi, double *coeff);
 
 /* ---- spectra --------------------------------------------------------- */
 /* in-place radix-2 FFT, sign=+1 inverse-direction (unnormalised) */

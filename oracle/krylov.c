@@ -330,8 +330,67 @@ void orc_init_disorder(const orc_params *p, const orc_site_params *sp,
             e0[i] = base;
         }
         if (sp->e_shift) e0[i] = e0[i] + sp->e_shift[i];
-        double sig = sp->ou_sigma ? sp->ou_sigma[i] : 0.0;
+        /* the stationary OU start (systemClass.cu:1036); the other models
+         * set the offset at every step */
+        double sig = (sp->noise_model == ORC_NOISE_OU && sp->ou_sigma) ? sp->ou_sigma[i] : 0.0;
         de[i] = sig * orc_normal(p->seed, traj, (uint64_t)i, 0, 2, 0);
+    }
+}
+
+double orc_zz_phase(uint64_t seed, uint64_t traj, uint64_t site, uint64_t k)
+{
+    uint64_t ctr[4] = {site, k >> 2, 5, 0}, key[2] = {seed, traj}, out[4];
+    orc_philox4x64_10(ctr, key, out);
+    return (double)(out[k & 3] >> 11) * (1.0 / 9007199254740992.0);
+}
+
+double orc_zz_cos(double u, double w2pi, double t)
+{
+    double x = u + w2pi * t;
+    x = x - floor(x);
+    if (x >= 1.0) x = 0.0;
+    double s, c;
+    orc_sincos_2pi(x, &s, &c);
+    return c;
+}
+
+void orc_zz_oscillators(int nosc, const double *omega_cm, const double *J, double domega,
+                        double temperature, double *w2pi, double *coeff)
+{
+    const double kB = 0.695035, conv = 5308.84, pi = 3.14159265358979323846;
+    for (int k = 0; k < nosc; ++k) {
+        double G = J[k] / (1 - exp(-1. / (kB * temperature) * omega_cm[k]));
+        coeff[k] = sqrt(2.) * sqrt(G * domega / pi);
+        w2pi[k] = omega_cm[k] / conv / (2 * pi);
+    }
+}
+
+void orc_noise_step(const orc_params *p, const orc_site_params *sp,
+                    uint64_t traj, int64_t step, double *de)
+{
+    switch (sp->noise_model) {
+    case ORC_NOISE_OU:
+        orc_ou_step(p, sp, traj, step, de);
+        break;
+    case ORC_NOISE_HSR:
+        for (int i = 0; i < p->n; ++i) {
+            double sig = sp->hsr_sigma ? sp->hsr_sigma[i] : 0.0;
+            de[i] = sig * orc_normal(p->seed, traj, (uint64_t)i, (uint64_t)step, 4, 0);
+        }
+        break;
+    case ORC_NOISE_ZZ: {
+        const double t = (double)step * p->dt_fs;
+        for (int i = 0; i < p->n; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < sp->zz_nosc; ++k)
+                acc = acc + sp->zz_coeff[k] * orc_zz_cos(orc_zz_phase(p->seed, traj, (uint64_t)i, (uint64_t)k),
+                                                         sp->zz_w2pi[k], t);
+            de[i] = acc;
+        }
+        break;
+    }
+    default:
+        for (int i = 0; i < p->n; ++i) de[i] = 0.0;
     }
 }
 
@@ -423,7 +482,7 @@ int orc_run(const orc_params *p, const orc_site_params *sp,
     if (pops && obs_every > 0) record(n, psi, group, ngroups, pops);
     if (corr && bra) cdot(n, bra, psi, &corr[0], &corr[1]);
     for (int64_t step = 0; step < nsteps; ++step) {
-        orc_ou_step(p, sp, traj, step, de);
+        orc_noise_step(p, sp, traj, step, de);
         for (int i = 0; i < n; ++i) diag[i] = de[i] + e0[i];
         for (int k = 0; k < p->krylov_substeps; ++k) {
             double err = krylov_cycle(n, rowptr, col, val, p->g, diag, &lo, tau,

@@ -31,10 +31,15 @@ class Params(C.Structure):
     ]
 
 
+NOISE_OU, NOISE_HSR, NOISE_ZZ, NOISE_NONE = 0, 1, 2, 3
+
+
 class SiteParams(C.Structure):
     _fields_ = [
         ("ou_tau", C.c_void_p), ("ou_sigma", C.c_void_p),
         ("e_site", C.c_void_p), ("e_shift", C.c_void_p),
+        ("noise_model", C.c_int), ("zz_nosc", C.c_int),
+        ("hsr_sigma", C.c_void_p), ("zz_w2pi", C.c_void_p), ("zz_coeff", C.c_void_p),
     ]
 
 
@@ -102,6 +107,12 @@ def lib():
                                   C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int, _dp,
                                   C.POINTER(C.c_double)]
         L.orc_set_adjoint_mode.argtypes = [C.c_int]
+        L.orc_noise_step.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), C.c_uint64, C.c_int64, _dp]
+        L.orc_zz_phase.restype = C.c_double
+        L.orc_zz_phase.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_zz_cos.restype = C.c_double
+        L.orc_zz_cos.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.orc_zz_oscillators.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
         _lib = L
     return _lib
 
@@ -244,12 +255,19 @@ def make_params(n, m, dt, K=1, S=1, reorth=REORTH_CGS, tol=1e-10, hbar=None, g=1
 
 
 class Sites:
-    """keeps the arrays alive for a SiteParams struct"""
+    """keeps the arrays alive for a SiteParams struct; noise = NOISE_OU (tau,
+    sigma), NOISE_HSR (hsr_sigma), NOISE_ZZ (zz_w2pi, zz_coeff) or NOISE_NONE"""
 
-    def __init__(self, n, tau=None, sigma=None, e_site=None, e_shift=None):
+    def __init__(self, n, tau=None, sigma=None, e_site=None, e_shift=None, noise=NOISE_OU,
+                 hsr_sigma=None, zz_w2pi=None, zz_coeff=None):
         f = lambda v: None if v is None else _c(np.broadcast_to(np.asarray(v, np.float64), (n,)).copy())
         self.tau, self.sigma, self.e_site, self.e_shift = f(tau), f(sigma), f(e_site), f(e_shift)
-        self.sp = SiteParams(_ptr(self.tau), _ptr(self.sigma), _ptr(self.e_site), _ptr(self.e_shift))
+        self.hsr = f(hsr_sigma)
+        self.zw = None if zz_w2pi is None else _c(np.asarray(zz_w2pi, np.float64))
+        self.zc = None if zz_coeff is None else _c(np.asarray(zz_coeff, np.float64))
+        nosc = 0 if self.zw is None else len(self.zw)
+        self.sp = SiteParams(_ptr(self.tau), _ptr(self.sigma), _ptr(self.e_site), _ptr(self.e_shift),
+                             noise, nosc, _ptr(self.hsr), _ptr(self.zw), _ptr(self.zc))
 
 
 def init_disorder(p, sites, traj):
@@ -263,6 +281,31 @@ def ou_step(p, sites, traj, step, de):
     de = _c(np.array(de, np.float64))
     lib().orc_ou_step(C.byref(p), C.byref(sites.sp), traj, step, de)
     return de
+
+
+def noise_step(p, sites, traj, step, de):
+    """the noise offset of step `step` for the Sites' noise model"""
+    de = _c(np.array(de, np.float64))
+    lib().orc_noise_step(C.byref(p), C.byref(sites.sp), traj, step, de)
+    return de
+
+
+def zz_phase(seed, traj, site, k):
+    return lib().orc_zz_phase(seed, traj, site, k)
+
+
+def zz_cos(u, w2pi, t):
+    return lib().orc_zz_cos(u, w2pi, t)
+
+
+def zz_oscillators(omega_cm, J, domega, temperature):
+    """(omega/2pi in 1/fs, amplitudes in cm^-1) of a sampled spectral density"""
+    om = _c(np.asarray(omega_cm, np.float64))
+    jj = _c(np.asarray(J, np.float64))
+    w = np.zeros(len(om))
+    c = np.zeros(len(om))
+    lib().orc_zz_oscillators(len(om), om, jj, domega, temperature, w, c)
+    return w, c
 
 
 def run(p, sites, rowptr, col, val, psi0, traj, nsteps, obs_every=0, group=None,
