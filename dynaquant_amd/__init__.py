@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdynaquant_amd.so")
 
 FP64, MIXED = 0, 1
+NOISE_OU, NOISE_HSR, NOISE_ZZ, NOISE_NONE = 0, 1, 2, 3
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "BREAKDOWN",
           5: "NONFINITE", 6: "SCALING_OVERFLOW", 7: "NO_DEVICE", 8: "STATE"}
@@ -27,6 +28,7 @@ EXPORTS = [
     "dq_get_error_sum", "dq_get_diagonal", "dq_spmv", "dq_pade_tbar", "dq_spectrum", "dq_fft",
     "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times", "dq_adjoint",
     "dq_get_tuning", "dq_set_tuning", "dq_autotune",
+    "dq_system_set_hsr_params", "dq_system_set_zz_params", "dq_zz_oscillators",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -43,7 +45,7 @@ class Config(C.Structure):
     _fields_ = [
         ("n_sites", C.c_int32), ("n_traj", C.c_int32), ("krylov_m", C.c_int32),
         ("krylov_substeps", C.c_int32), ("ou_substeps", C.c_int32), ("precision", C.c_int32),
-        ("force_batched", C.c_int32), ("reserved", C.c_int32),
+        ("force_batched", C.c_int32), ("noise_model", C.c_int32),
         ("dt_fs", C.c_double), ("hbar_cm_fs", C.c_double), ("coupling_scale", C.c_double),
         ("sigma_static", C.c_double), ("breakdown_tol", C.c_double), ("seed", C.c_uint64),
     ]
@@ -148,9 +150,9 @@ def tube(rings, radius, **kw):
 class System:
     def __init__(self, n, m, dt_fs, n_traj=1, krylov_substeps=1, ou_substeps=1, precision=FP64,
                  hbar=0.0, coupling_scale=1.0, sigma_static=0.0, breakdown_tol=0.0, seed=1,
-                 force_batched=False):
-        cfg = Config(n, n_traj, m, krylov_substeps, ou_substeps, precision, int(force_batched), 0,
-                     dt_fs, hbar, coupling_scale, sigma_static, breakdown_tol, seed)
+                 force_batched=False, noise_model=NOISE_OU):
+        cfg = Config(n, n_traj, m, krylov_substeps, ou_substeps, precision, int(force_batched),
+                     int(noise_model), dt_fs, hbar, coupling_scale, sigma_static, breakdown_tol, seed)
         self.cfg = cfg
         self.n, self.B, self.m = n, n_traj, m
         h = C.c_void_p()
@@ -185,6 +187,17 @@ class System:
         tau = _c(np.broadcast_to(tau_fs, (self.n,)), np.float64)
         sig = _c(np.broadcast_to(sigma, (self.n,)), np.float64)
         _check(lib().dq_system_set_ou_params(self._h, _p(tau), _p(sig)))
+
+    def set_hsr(self, sigma):
+        sig = _c(np.broadcast_to(sigma, (self.n,)), np.float64)
+        _check(lib().dq_system_set_hsr_params(self._h, _p(sig)))
+
+    def set_zz(self, omega_cm, coeff):
+        om = _c(omega_cm, np.float64)
+        co = _c(coeff, np.float64)
+        if om.shape != co.shape:
+            raise ValueError("omega and coeff differ in length")
+        _check(lib().dq_system_set_zz_params(self._h, len(om), _p(om), _p(co)))
 
     def set_trajectories(self, traj_ids=None, n_active=None):
         t = None if traj_ids is None else _c(traj_ids, np.uint64)
@@ -312,6 +325,16 @@ class System:
         sp = np.zeros(2 * nfft)
         _check(lib().dq_spectrum(self._h, C.c_double(sigma_fs), C.c_double(mu2), nfft, _p(om), _p(sp)))
         return om, sp.view(np.complex128)
+
+
+def zz_oscillators(omega_cm, J, domega, temperature):
+    """amplitudes (cm^-1) of a spectral density J(omega_k) sampled every domega"""
+    om = _c(omega_cm, np.float64)
+    jj = _c(J, np.float64)
+    out = np.zeros(len(om))
+    _check(lib().dq_zz_oscillators(len(om), _p(om), _p(jj), C.c_double(domega), C.c_double(temperature),
+                                   _p(out)))
+    return out
 
 
 def pade_tbar(m, alpha, beta, theta):

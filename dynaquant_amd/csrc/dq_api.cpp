@@ -68,6 +68,16 @@ struct dq_system {
     double last_ms = 0.0;
     int n = 0, B = 0, Bp = 0, m = 0, K = 1;
     bool have_h = false, have_ou = false, have_state = false, have_disorder = false;
+    bool have_hsr = false, have_zz = false;
+    bool noise_ready() const
+    {
+        switch (d.noise_model) {
+        case DQ_NOISE_OU: return have_ou;
+        case DQ_NOISE_HSR: return have_hsr;
+        case DQ_NOISE_ZZ: return have_zz;
+        default: return true;
+        }
+    }
     int64_t step = 0;                 // host mirror of the device counter
     int64_t obs_cap_rec = 0;          // records allocated
     std::vector<void*> allocs;        // everything owned
@@ -206,6 +216,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     if (cfg->ou_substeps < 1) return fail(DQ_ERR_INVALID_ARG, "ou_substeps must be >= 1");
     if (cfg->precision != DQ_PREC_FP64 && cfg->precision != DQ_PREC_MIXED)
         return fail(DQ_ERR_INVALID_ARG, "unknown precision");
+    if (cfg->noise_model < DQ_NOISE_OU || cfg->noise_model > DQ_NOISE_NONE)
+        return fail(DQ_ERR_INVALID_ARG, "unknown noise model");
     if (!(cfg->dt_fs > 0.0)) return fail(DQ_ERR_INVALID_ARG, "dt_fs must be > 0");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(DQ_ERR_NO_DEVICE, "no HIP device");
@@ -245,6 +257,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.S = cfg->ou_substeps;
+    d.noise_model = cfg->noise_model;
+    d.dt = cfg->dt_fs;
     d.seed = cfg->seed;
     // systemClass.cu:993-994: subdt = dt / S, sqrt(subdt)
     d.subdt = cfg->dt_fs / (double)cfg->ou_substeps;
@@ -412,6 +426,71 @@ dq_status dq_system_set_ou_params(dq_system* sys, const double* tau_fs, const do
     return DQ_OK;
 }
 
+dq_status dq_system_set_hsr_params(dq_system* sys, const double* sigma)
+{
+    if (!sys || !sigma) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    const int n = sys->n;
+    for (int i = 0; i < n; ++i)
+        if (!(sigma[i] >= 0.0) || !std::isfinite(sigma[i])) return fail(DQ_ERR_INVALID_ARG, "sigma must be finite and >= 0");
+    Dev& d = sys->d;
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->release(*(double**)&d.hsr_sigma);
+    double* a;
+    DQ_TRY(sys->alloc(&a, n));
+    HIP_TRY(hipMemcpyAsync(a, sigma, sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.hsr_sigma = a;
+    sys->have_hsr = true;
+    return DQ_OK;
+}
+
+dq_status dq_system_set_zz_params(dq_system* sys, int32_t nosc, const double* omega_cm, const double* coeff)
+{
+    if (!sys || !omega_cm || !coeff) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (nosc < 1) return fail(DQ_ERR_INVALID_ARG, "nosc must be >= 1");
+    // includeFiles.cu:16 (INVFS_TO_INVCM): omega in rad/fs = omega_cm / 5308.84
+    const double conv = 5308.84, pi = 3.14159265358979323846;
+    std::vector<double> w(nosc);
+    for (int k = 0; k < nosc; ++k) {
+        if (!std::isfinite(omega_cm[k]) || !std::isfinite(coeff[k]) || omega_cm[k] < 0.0)
+            return fail(DQ_ERR_INVALID_ARG, "omega must be finite and >= 0, coeff finite");
+        w[k] = omega_cm[k] / conv / (2 * pi);
+    }
+    Dev& d = sys->d;
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    sys->release(*(double**)&d.zz_w2pi);
+    sys->release(*(double**)&d.zz_coeff);
+    double *a, *c;
+    DQ_TRY(sys->alloc(&a, nosc));
+    DQ_TRY(sys->alloc(&c, nosc));
+    HIP_TRY(hipMemcpyAsync(a, w.data(), sizeof(double) * nosc, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(c, coeff, sizeof(double) * nosc, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.zz_w2pi = a;
+    d.zz_coeff = c;
+    d.zz_nosc = nosc;
+    sys->have_zz = true;
+    return DQ_OK;
+}
+
+dq_status dq_zz_oscillators(int32_t nosc, const double* omega_cm, const double* J, double domega,
+                            double temperature_K, double* coeff)
+{
+    if (!omega_cm || !J || !coeff) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (nosc < 1 || !(domega > 0.0) || !(temperature_K > 0.0))
+        return fail(DQ_ERR_INVALID_ARG, "nosc >= 1, domega > 0 and temperature > 0 required");
+    // systemClass.cu:1118-1145 (hbar = 1 in cm^-1 units), includeFiles.cu:13
+    const double kB = 0.695035, pi = 3.14159265358979323846;
+    for (int k = 0; k < nosc; ++k) {
+        const double G = J[k] / (1 - std::exp(-1. / (kB * temperature_K) * omega_cm[k]));
+        coeff[k] = std::sqrt(2.) * std::sqrt(G * domega / pi);
+        if (!std::isfinite(coeff[k])) return fail(DQ_ERR_NONFINITE, "non-finite amplitude (omega = 0 or J < 0?)");
+    }
+    return DQ_OK;
+}
+
 dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, const int32_t* n_active)
 {
     if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
@@ -454,7 +533,7 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
 dq_status dq_system_reset_disorder(dq_system* sys)
 {
     if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
-    if (!sys->have_h || !sys->have_ou) return fail(DQ_ERR_STATE, "set the Hamiltonian and OU parameters first");
+    if (!sys->have_h || !sys->noise_ready()) return fail(DQ_ERR_STATE, "set the Hamiltonian and noise parameters first");
     Dev& d = sys->d;
     HIP_TRY(dq::launch_init_disorder(d, sys->stream));
     HIP_TRY(hipMemsetAsync(d.err_sum, 0, sizeof(double) * d.Bp, sys->stream));
@@ -683,7 +762,7 @@ static hipError_t enqueue_finalize(dq_system* sys)
 static dq_status check_ready(dq_system* sys)
 {
     if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
-    if (!sys->have_h || !sys->have_ou) return fail(DQ_ERR_STATE, "Hamiltonian/OU parameters not set");
+    if (!sys->have_h || !sys->noise_ready()) return fail(DQ_ERR_STATE, "Hamiltonian/noise parameters not set");
     if (!sys->have_disorder) return fail(DQ_ERR_STATE, "call dq_system_reset_disorder first");
     if (!sys->have_state) return fail(DQ_ERR_STATE, "call dq_system_set_state first");
     return DQ_OK;

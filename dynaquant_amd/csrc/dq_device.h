@@ -152,7 +152,28 @@ __host__ __device__ inline void box_muller(uint64_t a, uint64_t b, double* z0, d
     *z1 = r * s;
 }
 
-enum : uint64_t { kStreamStatic = 1, kStreamOuInit = 2, kStreamOu = 3 };
+enum : uint64_t { kStreamStatic = 1, kStreamOuInit = 2, kStreamOu = 3, kStreamHsr = 4, kStreamZz = 5 };
+
+// uniform phases u in [0,1) of the Zhong-Zhao oscillators 4q..4q+3 of a site
+// (fixed per trajectory: the counter has no step)
+__host__ __device__ inline void zz_phase4(uint64_t seed, uint64_t traj, uint64_t site, uint64_t q, double u[4])
+{
+    uint64_t c[4] = {site, q, kStreamZz, 0};
+    philox4x64_10(c, seed, traj);
+    for (int r = 0; r < 4; ++r) u[r] = (double)(c[r] >> 11) * 0x1p-53;
+}
+
+// cos(2 pi (u + w t)) with the argument reduced to [0,1)
+__host__ __device__ inline double zz_cos(double u, double w2pi, double t)
+{
+#pragma clang fp contract(off)
+    double x = u + w2pi * t;
+    x = x - __builtin_floor(x);
+    if (x >= 1.0) x = 0.0;
+    double s, c;
+    rng_sincos_2pi(x, &s, &c);
+    return c;
+}
 
 // four normals of counter (site, step, stream, block)
 __host__ __device__ inline void normal4(uint64_t seed, uint64_t traj, uint64_t site, uint64_t step,

@@ -29,6 +29,12 @@ struct Dev {
     int spmm_pipe;                  // batched SpMM: 1 = two gather buffers in flight
     int s0, sn;                     // batched kernels work on trajectory slots [s0, s0 + sn)
     int S;                          // OU substeps
+    int noise_model;                // DQ_NOISE_*: 0 OU, 1 HSR, 2 ZZ real, 3 none
+    int zz_nosc;                    // ZZ oscillators per site
+    double dt;                      // fs (ZZ time t = step dt)
+    const double* hsr_sigma;        // [n] HSR standard deviations
+    const double* zz_w2pi;          // [zz_nosc] omega / 2 pi, 1/fs
+    const double* zz_coeff;         // [zz_nosc] cm^-1
     uint64_t seed;
     double subdt, sqrt_subdt;
     double g;                       // coupling scale

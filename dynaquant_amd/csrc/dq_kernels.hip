@@ -169,8 +169,10 @@ __global__ void k_init_disorder(Dev d)
             } else {
                 e0 = base;
             }
-            normal4(d.seed, d.traj_id[b], (uint64_t)i, 0, kStreamOuInit, 0, z);
-            de = d.ou_sigma[i] * z[0];
+            if (d.noise_model == 0) {               // stationary OU start (systemClass.cu:1036)
+                normal4(d.seed, d.traj_id[b], (uint64_t)i, 0, kStreamOuInit, 0, z);
+                de = d.ou_sigma[i] * z[0];
+            }
         }
         d.e0[e] = e0;
         d.de[e] = de;
@@ -248,15 +250,38 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
         }
         if (noise && i < nact) {
             double cur = d.de[e];
-            const double invTau = d.inv_tau[i], coeff = d.coeff[i];
-            if (coeff != 0.0 || invTau != 0.0) {
-                for (int s = 0; s < d.S; ++s) {
-                    double z[4];
-                    normal4(d.seed, tid_b, (uint64_t)i, (uint64_t)step, kStreamOu, (uint64_t)(s >> 2), z);
-                    double xi = z[s & 3];
-                    cur = cur - cur * invTau * d.subdt + coeff * xi * d.sqrt_subdt;
+            if (d.noise_model == 0) {
+                // Kubo-Anderson / OU, S Euler-Maruyama substeps (gpuSysHelper.cu:73-128)
+                const double invTau = d.inv_tau[i], coeff = d.coeff[i];
+                if (coeff != 0.0 || invTau != 0.0) {
+                    for (int s = 0; s < d.S; ++s) {
+                        double z[4];
+                        normal4(d.seed, tid_b, (uint64_t)i, (uint64_t)step, kStreamOu, (uint64_t)(s >> 2), z);
+                        double xi = z[s & 3];
+                        cur = cur - cur * invTau * d.subdt + coeff * xi * d.sqrt_subdt;
+                    }
+                    d.de[e] = cur;
                 }
+            } else if (d.noise_model == 1) {
+                // Haken-Strobl-Reineker white noise (gpuSysHelper.cu:32-66)
+                double z[4];
+                normal4(d.seed, tid_b, (uint64_t)i, (uint64_t)step, kStreamHsr, 0, z);
+                cur = d.hsr_sigma[i] * z[0];
                 d.de[e] = cur;
+            } else if (d.noise_model == 2) {
+                // Zhong-Zhao real noise: sum_k c_k cos(omega_k t + phi_k) (gpuSysHelper.cu:134-177)
+                const double t = (double)step * d.dt;
+                double acc = 0.0;
+                for (int q = 0; q < d.zz_nosc; q += 4) {
+                    double u[4];
+                    zz_phase4(d.seed, tid_b, (uint64_t)i, (uint64_t)(q >> 2), u);
+                    for (int r = 0; r < 4 && q + r < d.zz_nosc; ++r)
+                        acc = acc + d.zz_coeff[q + r] * zz_cos(u[r], d.zz_w2pi[q + r], t);
+                }
+                cur = acc;
+                d.de[e] = cur;
+            } else {
+                cur = 0.0;                          // constant Hamiltonian
             }
             diag[e] = (VT)(cur + d.e0[e]);
         }

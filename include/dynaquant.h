@@ -47,6 +47,19 @@ typedef enum dq_precision {
                            reorthogonalisation sums, Pade */
 } dq_precision;
 
+/* Site-energy noise on the diagonal, updated before every step
+ * (systemClass::updateHamiltonian, systemClass.cu:1881-2013). */
+typedef enum dq_noise {
+    DQ_NOISE_OU = 0,    /* Kubo-Anderson / Ornstein-Uhlenbeck, dq_system_set_ou_params
+                           (SIM_TYPE_KUBO_ANDERSON) */
+    DQ_NOISE_HSR = 1,   /* Haken-Strobl-Reineker white noise, dq_system_set_hsr_params
+                           (SIM_TYPE_HSR) */
+    DQ_NOISE_ZZ = 2,    /* Zhong-Zhao real noise, dq_system_set_zz_params
+                           (SIM_TYPE_ZZ_REAL_NOISE) */
+    DQ_NOISE_NONE = 3   /* constant Hamiltonian apart from static disorder
+                           (SIM_TYPE_CONST_HAM) */
+} dq_noise;
+
 typedef struct dq_system dq_system;
 
 /* infile.in keys of the reference in brackets (systemClass.cu:231-630) */
@@ -58,7 +71,7 @@ typedef struct dq_config {
     int32_t ou_substeps;      /* [ka_substeps] Euler-Maruyama substeps per dt */
     int32_t precision;        /* dq_precision */
     int32_t force_batched;    /* 1: run B=1 through the batched kernels */
-    int32_t reserved;
+    int32_t noise_model;      /* [simtype] dq_noise, 0 = OU */
     double dt_fs;             /* [dt] */
     double hbar_cm_fs;        /* <= 0: the reference's 5308.84 (cppUtility.cpp:10-35) */
     double coupling_scale;    /* g: H = diag + g * couplings */
@@ -84,6 +97,22 @@ dq_status dq_system_set_hamiltonian_csr(dq_system *sys, int32_t n, const int32_t
 /* Per-site OU relaxation time (fs) and standard deviation (cm^-1)
  * (ka_paramsfile lines "tau sigma", systemClass.cu:980-1071). */
 dq_status dq_system_set_ou_params(dq_system *sys, const double *tau_fs, const double *sigma);
+/* HSR: per-site standard deviation (cm^-1) of a fresh Gaussian offset every
+ * step (hsr parameter file, systemClass.cu:910-975; gpuSysHelper.cu:32-66;
+ * Philox stream 4 instead of the shared MRG32k3a states). */
+dq_status dq_system_set_hsr_params(dq_system *sys, const double *sigma);
+/* ZZ real noise: offset_i(t) = sum_k coeff[k] cos(omega_k t + phi_ik), t = step
+ * dt, omega_k = omega_cm[k] / 5308.84 rad/fs (includeFiles.cu:16), phases
+ * uniform in [0, 2 pi) per site and trajectory from Philox stream 5 (the
+ * reference draws them with Boost per run, systemClass.cu:1075-1243;
+ * gpuSysHelper.cu:134-177).  coeff from dq_zz_oscillators. */
+dq_status dq_system_set_zz_params(dq_system *sys, int32_t nosc, const double *omega_cm,
+                                  const double *coeff);
+/* Amplitudes of a spectral density J(omega_k) sampled every domega cm^-1 at
+ * temperature T (K): G = J / (1 - exp(-omega / (kB T))), coeff = sqrt(2 G domega
+ * / pi), kB = 0.695035 cm^-1/K (systemClass.cu:1118-1145). */
+dq_status dq_zz_oscillators(int32_t nosc, const double *omega_cm, const double *J, double domega,
+                            double temperature_K, double *coeff);
 /* Philox key word 1 of each trajectory (NULL: 0..B-1) and its active site
  * count (NULL: all N).  A trajectory with n_active < N propagates under the
  * leading n_active x n_active block of H.  Clears observables and the state

@@ -45,3 +45,15 @@ def test_tube_generator_runs_on_host(engine):
     pos, dip, wall, rp, col, val = engine.tube([10], [3.0])
     assert len(wall) == 600 and len(col) == 42720
     assert np.allclose(np.linalg.norm(dip, axis=1), 10.0)
+
+
+def test_zz_oscillators_host_helper_matches_oracle(engine, oracle):
+    """the engine's spectral-density amplitudes are the oracle's (host code only)"""
+    om = 8.0 * np.arange(1, 101)
+    J = 2 * 35.0 * om * 160.0 / (om ** 2 + 160.0 ** 2)
+    for T in (77.0, 300.0):
+        c = engine.zz_oscillators(om, J, 8.0, T)
+        w, co = oracle.zz_oscillators(om, J, 8.0, T)
+        assert np.array_equal(c, co)
+    with pytest.raises(engine.DQError):
+        engine.zz_oscillators(om, J, 8.0, 0.0)
