@@ -29,6 +29,8 @@ EXPORTS = [
     "dq_last_run_ms", "dq_set_profiling", "dq_get_kernel_times", "dq_adjoint",
     "dq_get_tuning", "dq_set_tuning", "dq_autotune",
     "dq_system_set_hsr_params", "dq_system_set_zz_params", "dq_zz_oscillators",
+    "dq_read_hamiltonian_file", "dq_read_state_file", "dq_read_dipole_file", "dq_read_site_columns",
+    "dq_read_spectral_density", "dq_parse_infile", "dq_write_pops_file", "dq_read_pops_file",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -335,6 +337,102 @@ def zz_oscillators(omega_cm, J, domega, temperature):
     _check(lib().dq_zz_oscillators(len(om), _p(om), _p(jj), C.c_double(domega), C.c_double(temperature),
                                    _p(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# files of the reference's workflow
+# ---------------------------------------------------------------------------
+class Infile(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("temperature", C.c_double), ("sigma_static", C.c_double),
+        ("gauss_width", C.c_double), ("total_steps", C.c_int32), ("ensemble_runs", C.c_int32),
+        ("ensemble_mode", C.c_int32), ("m", C.c_int32), ("ka_substeps", C.c_int32),
+        ("noise_model", C.c_int32), ("prop_mode", C.c_int32), ("static_dis", C.c_int32),
+        ("absorb_spec", C.c_int32), ("cd_spec", C.c_int32), ("lineshape_dress", C.c_int32),
+        ("state_looping", C.c_int32), ("steps_per_loop", C.c_int32), ("write_pops", C.c_int32),
+        ("print_all_pops", C.c_int32), ("write_binary_pops", C.c_int32), ("write_states", C.c_int32),
+        ("print_partic", C.c_int32), ("debug_diags", C.c_int32), ("print_pops", C.c_int32 * 10),
+        ("init_state", C.c_int32), ("init_from_file", C.c_int32),
+        ("ham_file", C.c_char * 512), ("dipole_file", C.c_char * 512), ("init_state_file", C.c_char * 512),
+        ("ka_params_file", C.c_char * 512), ("hsr_file", C.c_char * 512), ("zz_specd_file", C.c_char * 512),
+        ("out_folder", C.c_char * 512),
+    ]
+
+
+def parse_infile(path):
+    r = Infile()
+    _check(lib().dq_parse_infile(os.fsencode(path), C.byref(r)))
+    out = {}
+    for name, _ in Infile._fields_:
+        v = getattr(r, name)
+        if isinstance(v, bytes):
+            v = v.decode()
+        elif name == "print_pops":
+            v = list(v)
+        out[name] = v
+    return out
+
+
+def read_hamiltonian(path):
+    """(n, rowptr, col, val, e_site) of a Hamiltonian file"""
+    n = C.c_int32()
+    nnz = C.c_int64()
+    _check(lib().dq_read_hamiltonian_file(os.fsencode(path), C.byref(n), C.byref(nnz), None, None, None, None))
+    rp = np.zeros(n.value + 1, np.int32)
+    col = np.zeros(max(nnz.value, 1), np.int32)
+    val = np.zeros(max(nnz.value, 1))
+    es = np.zeros(n.value)
+    _check(lib().dq_read_hamiltonian_file(os.fsencode(path), C.byref(n), C.byref(nnz), _p(rp), _p(col), _p(val),
+                                          _p(es)))
+    return n.value, rp, col[:nnz.value], val[:nnz.value], es
+
+
+def read_state(path, n):
+    a = np.zeros(2 * n)
+    _check(lib().dq_read_state_file(os.fsencode(path), int(n), _p(a)))
+    return a.view(np.complex128)
+
+
+def read_dipoles(path, n):
+    a = np.zeros(3 * n)
+    _check(lib().dq_read_dipole_file(os.fsencode(path), int(n), _p(a)))
+    return a.reshape(n, 3)
+
+
+def read_site_columns(path, n, ncol):
+    a = np.zeros(n * ncol)
+    _check(lib().dq_read_site_columns(os.fsencode(path), int(n), int(ncol), _p(a)))
+    return a.reshape(n, ncol)
+
+
+def read_spectral_density(path):
+    """(domega, omega[nosc], J[nosc]) in cm^-1"""
+    k = C.c_int32()
+    dw = C.c_double()
+    _check(lib().dq_read_spectral_density(os.fsencode(path), C.byref(k), C.byref(dw), None, None))
+    om = np.zeros(k.value)
+    jj = np.zeros(k.value)
+    _check(lib().dq_read_spectral_density(os.fsencode(path), C.byref(k), C.byref(dw), _p(om), _p(jj)))
+    return dw.value, om, jj
+
+
+def write_pops(path, pops, dt, nsims=1, binary=False):
+    """write (or continue) a population file; pops[t-1][site] for steps 1..T"""
+    p = _c(pops, np.float64)
+    T, P = p.shape
+    _check(lib().dq_write_pops_file(os.fsencode(path), int(binary), C.c_double(dt), T, P, _p(p), int(nsims)))
+
+
+def read_pops(path, binary=False):
+    """(nsims, dt, pops[T][P])"""
+    ns, T, P = C.c_int32(), C.c_int32(), C.c_int32()
+    dt = C.c_double()
+    _check(lib().dq_read_pops_file(os.fsencode(path), int(binary), C.byref(ns), C.byref(dt), C.byref(T), C.byref(P),
+                                   None))
+    out = np.zeros(T.value * P.value)
+    _check(lib().dq_read_pops_file(os.fsencode(path), int(binary), C.byref(ns), C.byref(dt), C.byref(T), C.byref(P),
+                                   _p(out)))
+    return ns.value, dt.value, out.reshape(T.value, P.value)
 
 
 def pade_tbar(m, alpha, beta, theta):

@@ -55,6 +55,10 @@ double reference_hbar()
 
 }  // namespace
 
+namespace dq {
+dq_status set_error(dq_status st, const std::string& msg) { return fail(st, msg); }
+}
+
 struct dq_system {
     dq_config cfg{};
     Dev d{};

@@ -272,6 +272,72 @@ dq_status dq_set_tuning(dq_system *sys, const dq_tuning *t);
  * first and, after a search, appended.  Needs set_hamiltonian. */
 dq_status dq_autotune(dq_system *sys, int32_t reps, const char *cache_path, dq_tuning *chosen);
 
+/* ---- files of the reference's workflow (host only) -------------------- */
+/* Hamiltonian file (sparseMatrixClasses.cu:74-267): line 1 "%%MatrixMarket
+ * matrix coordinate real general|symmetric|hermitian", line 2 BASE_ZERO or
+ * BASE_ONE, line 3 "N N nnz", then nnz lines "row col value".  Symmetric and
+ * hermitian files hold one triangle (the other is mirrored); a general file
+ * must be symmetric.  Diagonal entries go to e_site[N], off-diagonal ones to a
+ * column-sorted CSR (duplicates summed), ready for
+ * dq_system_set_hamiltonian_csr.  Call with rowptr == NULL for *n_out and
+ * *nnz_out (off-diagonal count) first. */
+dq_status dq_read_hamiltonian_file(const char *path, int32_t *n_out, int64_t *nnz_out, int32_t *rowptr,
+                                   int32_t *col, double *val, double *e_site);
+/* Initial state file (systemClass.cu:1383-1481): BASE_ZERO|BASE_ONE, one
+ * comment line, "index coefficient" lines up to END or the end of the file;
+ * real coefficients (read as double), normalised; psi[2n] interleaved complex. */
+dq_status dq_read_state_file(const char *path, int32_t n, double *psi);
+/* Transition dipoles (spectraClass.cu:173-262): n lines "mu_x mu_y mu_z"; mu[3n]. */
+dq_status dq_read_dipole_file(const char *path, int32_t n, double *mu);
+/* Per-site parameter columns: ka_paramsfile "tau sigma" (ncol 2,
+ * systemClass.cu:980-1071) or hsr_stddevfile "sigma" (ncol 1,
+ * systemClass.cu:910-975); out[n][ncol]. */
+dq_status dq_read_site_columns(const char *path, int32_t n, int32_t ncol, double *out);
+/* Spectral density (systemClass.cu:1090-1160): "domega nosc", then nosc lines
+ * "omega J" (cm^-1).  omega/J may be NULL to query *nosc_out. */
+dq_status dq_read_spectral_density(const char *path, int32_t *nosc_out, double *domega_out, double *omega,
+                                   double *J);
+
+/* infile.in keys used by the propagation (systemClass.cu:183-735).  Lines
+ * "key = value"; a key containing '!' is a comment.  (The reference drops the
+ * character after '=', so it needs "key = value" with a blank; both forms
+ * read the same here.) */
+typedef struct dq_infile {
+    double dt;                  /* dt (fs) */
+    double temperature;         /* temperature (K), ZZ noise */
+    double sigma_static;        /* sigmastatdis (cm^-1) */
+    double gauss_width;         /* gausswidth (fs) */
+    int32_t total_steps;        /* totalsteps */
+    int32_t ensemble_runs;      /* ensembleruns */
+    int32_t ensemble_mode;      /* ensembleavgmode: 0 NONE, 1 FIRST, 2 CONTINUATION */
+    int32_t m;                  /* m */
+    int32_t ka_substeps;        /* ka_substeps */
+    int32_t noise_model;        /* simtype as dq_noise (CONST_HAM -> DQ_NOISE_NONE) */
+    int32_t prop_mode;          /* propmode: 0 LANCZOS, 1 TAYLOR_KRYLOV */
+    int32_t static_dis;         /* staticdis */
+    int32_t absorb_spec, cd_spec, lineshape_dress;
+    int32_t state_looping, steps_per_loop;
+    int32_t write_pops, print_all_pops, write_binary_pops, write_states, print_partic, debug_diags;
+    int32_t print_pops[10];     /* printpop0..printpop9, -1 = unused */
+    int32_t init_state, init_from_file;
+    char ham_file[512], dipole_file[512], init_state_file[512], ka_params_file[512], hsr_file[512],
+        zz_specd_file[512], out_folder[512];
+} dq_infile;
+dq_status dq_parse_infile(const char *path, dq_infile *out);
+
+/* Population file (popPrintClass.cu:159-335, 442-522), text or binary.  Text:
+ * "numsims S", "dt", "totalsteps T", "numprintedsites P", then for t = 1..T a
+ * "STEP t" line and P lines "%25.17g (site)".  Binary: int32 S, double dt,
+ * int32 T, int32 P, T*P doubles.  pops[t-1][site] are population sums over S
+ * runs.  Writing to an existing file continues it (ENS_AVG_MODE_CONTINUATION):
+ * its populations and run count are added after checking dt, T and P, and
+ * the result replaces it through a temporary file. */
+dq_status dq_write_pops_file(const char *path, int32_t binary, double dt, int32_t total_steps,
+                             int32_t nsites, const double *pops, int32_t nsims);
+/* pops may be NULL to read the header only */
+dq_status dq_read_pops_file(const char *path, int32_t binary, int32_t *nsims, double *dt,
+                            int32_t *total_steps, int32_t *nsites, double *pops);
+
 #ifdef __cplusplus
 }
 #endif
