@@ -312,7 +312,7 @@ typedef struct dq_infile {
     int32_t ensemble_mode;      /* ensembleavgmode: 0 NONE, 1 FIRST, 2 CONTINUATION */
     int32_t m;                  /* m */
     int32_t ka_substeps;        /* ka_substeps */
-    int32_t noise_model;        /* simtype as dq_noise (CONST_HAM -> DQ_NOISE_NONE) */
+    int32_t noise_model;        /* simtype as a dq_noise value; CONST_HAM gives DQ_NOISE_NONE */
     int32_t prop_mode;          /* propmode: 0 LANCZOS, 1 TAYLOR_KRYLOV */
     int32_t static_dis;         /* staticdis */
     int32_t absorb_spec, cd_spec, lineshape_dress;
