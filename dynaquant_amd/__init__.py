@@ -31,6 +31,7 @@ EXPORTS = [
     "dq_system_set_hsr_params", "dq_system_set_zz_params", "dq_zz_oscillators",
     "dq_read_hamiltonian_file", "dq_read_state_file", "dq_read_dipole_file", "dq_read_site_columns",
     "dq_read_spectral_density", "dq_parse_infile", "dq_write_pops_file", "dq_read_pops_file",
+    "dq_read_position_file", "dq_cd_vectors", "dq_system_set_participation", "dq_get_participation",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -227,6 +228,10 @@ class System:
         _check(lib().dq_system_set_observables(self._h, _p(g), ngroups, every, max_steps, _p(br)))
         self.every, self.ngroups = every, ngroups
 
+    def set_participation(self, on=True):
+        """record 1 / sum_i |psi_i|^4 every step (after set_observables)"""
+        _check(lib().dq_system_set_participation(self._h, int(on)))
+
     def update_noise(self):
         _check(lib().dq_update_noise(self._h))
 
@@ -306,6 +311,13 @@ class System:
         _check(lib().dq_get_corr(self._h, _p(out)))
         return out.view(np.complex128).reshape(self.B, T + 1)
 
+    def participation(self):
+        """pr[b][n] = 1 / sum_i |psi_b,i(n)|^4, n = 0..step"""
+        T = self.get_step()
+        out = np.zeros(self.B * (T + 1))
+        _check(lib().dq_get_participation(self._h, _p(out)))
+        return out.reshape(self.B, T + 1)
+
     def error_sum(self):
         out = np.zeros(self.B)
         _check(lib().dq_get_error_sum(self._h, _p(out)))
@@ -355,7 +367,7 @@ class Infile(C.Structure):
         ("init_state", C.c_int32), ("init_from_file", C.c_int32),
         ("ham_file", C.c_char * 512), ("dipole_file", C.c_char * 512), ("init_state_file", C.c_char * 512),
         ("ka_params_file", C.c_char * 512), ("hsr_file", C.c_char * 512), ("zz_specd_file", C.c_char * 512),
-        ("out_folder", C.c_char * 512),
+        ("out_folder", C.c_char * 512), ("pos_file", C.c_char * 512),
     ]
 
 
@@ -397,6 +409,46 @@ def read_dipoles(path, n):
     a = np.zeros(3 * n)
     _check(lib().dq_read_dipole_file(os.fsencode(path), int(n), _p(a)))
     return a.reshape(n, 3)
+
+
+def read_positions(path, n):
+    a = np.zeros(3 * n)
+    _check(lib().dq_read_position_file(os.fsencode(path), int(n), _p(a)))
+    return a.reshape(n, 3)
+
+
+CD_PREFACTOR = np.pi / 2 * 1e-7
+
+
+def cd_vectors(pos, mu):
+    """(rd[n][3] = pos x mu per site, mu_tot[3], rd_tot[3]) on the host"""
+    pos = _c(pos, np.float64)
+    mu = _c(mu, np.float64)
+    n = pos.shape[0]
+    if pos.shape != (n, 3) or mu.shape != (n, 3):
+        raise ValueError("pos and mu must both be [n][3]")
+    rd = np.zeros((n, 3))
+    mt = np.zeros(3)
+    rt = np.zeros(3)
+    _check(lib().dq_cd_vectors(n, _p(pos), _p(mu), _p(rd), _p(mt), _p(rt)))
+    return rd, mt, rt
+
+
+def spectrum_runs(pos, mu, kind="absorption"):
+    """the three direction runs of an isotropic spectrum: a list of
+    (psi0, bra, scale) for d = x, y, z, where the run propagates psi0, records
+    C(n) = <bra|psi(n)> and dq_spectrum takes `scale` (spectraClass.cu:437-496,
+    641-648); the isotropic spectrum is the mean of the three."""
+    mu = _c(mu, np.float64)
+    if kind == "absorption":
+        mt = np.sqrt((mu * mu).sum(0))
+        return [((mu[:, d] / mt[d]).astype(np.complex128), (mu[:, d] / mt[d]).astype(np.complex128),
+                 mt[d] ** 2) for d in range(3)]
+    if kind != "cd":
+        raise ValueError("kind is 'absorption' or 'cd'")
+    rd, mt, rt = cd_vectors(pos, mu)
+    return [((rd[:, d] / rt[d]).astype(np.complex128), (mu[:, d] / mt[d]).astype(np.complex128),
+             CD_PREFACTOR * mt[d] * rt[d]) for d in range(3)]
 
 
 def read_site_columns(path, n, ncol):

@@ -295,7 +295,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     ALLOC(&d.err_sum, (size_t)d.Bp);
     ALLOC(&d.broke, (size_t)d.Bp);
     ALLOC(&d.status, 1);
-    size_t nq = std::max<size_t>(2 * (s->m + 1) + 1, dq::kMaxGroups + 2);
+    size_t nq = std::max<size_t>(2 * (s->m + 1) + 1, dq::kMaxGroups + 3);
     // partial sums sized for the smallest tiles the tuning allows
     const size_t te_max = plain ? (size_t)d.ntile_e : (size_t)((d.n + dq::kRowsMin - 1) / dq::kRowsMin);
     const size_t ts_max = plain ? (size_t)d.ntile_s : (size_t)((d.n + 4 * dq::kRBMin - 1) / (4 * dq::kRBMin));
@@ -613,6 +613,7 @@ dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_
     sys->release(d.pops);
     sys->release(*(double2**)&d.bra);
     sys->release(d.corr);
+    sys->release(d.partic);
     const int64_t nrec = every > 0 ? max_steps / every + 1 : 0;
     if (every > 0) {
         int32_t* gp;
@@ -639,6 +640,17 @@ dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_
     sys->max_steps = max_steps;
     sys->obs_cap_rec = nrec;
     sys->have_bra = bra != nullptr;
+    return DQ_OK;
+}
+
+dq_status dq_system_set_participation(dq_system* sys, int32_t on)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    Dev& d = sys->d;
+    sys->drop_graph();
+    hipStreamSynchronize(sys->stream);
+    sys->release(d.partic);
+    if (on) DQ_TRY(sys->alloc(&d.partic, (size_t)(sys->max_steps + 1) * d.Bp));
     return DQ_OK;
 }
 
@@ -713,7 +725,7 @@ static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_
 static hipError_t enqueue_cycle(dq_system* sys, bool first_in_step, bool last_in_step)
 {
     const Dev& d = sys->d;
-    return enqueue_cycle_dev(sys, d, first_in_step, last_in_step, d.every > 0 || d.bra != nullptr);
+    return enqueue_cycle_dev(sys, d, first_in_step, last_in_step, dq::obs_on(d));
 }
 
 static hipError_t enqueue_step(dq_system* sys)
@@ -732,7 +744,7 @@ static hipError_t enqueue_step(dq_system* sys)
 static hipError_t enqueue_step_slices(dq_system* sys)
 {
     const Dev& d = sys->d;
-    const bool obs = d.every > 0 || d.bra != nullptr;
+    const bool obs = dq::obs_on(d);
     const int ns = sys->slices;
     hipError_t e;
     if ((e = hipEventRecord(sys->ev_fork, sys->stream)) != hipSuccess) return e;
@@ -756,7 +768,7 @@ static hipError_t enqueue_step_slices(dq_system* sys)
 static hipError_t enqueue_finalize(dq_system* sys)
 {
     const Dev& d = sys->d;
-    const bool obs = (d.every > 0 || d.bra != nullptr);
+    const bool obs = dq::obs_on(d);
     hipError_t e = dq::launch_prologue(d, 1, 0, obs, 0, sys->stream);
     if (e != hipSuccess) return e;
     if (obs && (e = dq::launch_reduce_obs(d, sys->stream)) != hipSuccess) return e;
@@ -790,7 +802,7 @@ dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
 {
     DQ_TRY(check_ready(sys));
     if (nsteps < 0) return fail(DQ_ERR_INVALID_ARG, "nsteps < 0");
-    if ((sys->every > 0 || sys->have_bra) && sys->step + nsteps > sys->max_steps)
+    if (dq::obs_on(sys->d) && sys->step + nsteps > sys->max_steps)
         return fail(DQ_ERR_INVALID_ARG, "run exceeds max_steps of the observables");
     hipStream_t st = sys->stream;
     if (sys->profiling) use_graph = 0;
@@ -892,6 +904,22 @@ dq_status dq_get_corr(dq_system* sys, double* corr)
             corr[((size_t)b * (T + 1) + k) * 2] = h[(size_t)k * d.Bp + sb].x;
             corr[((size_t)b * (T + 1) + k) * 2 + 1] = h[(size_t)k * d.Bp + sb].y;
         }
+    }
+    return DQ_OK;
+}
+
+dq_status dq_get_participation(dq_system* sys, double* pr)
+{
+    if (!sys || !pr) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    const Dev& d = sys->d;
+    if (!d.partic) return fail(DQ_ERR_STATE, "participation ratio not enabled");
+    const int64_t T = std::min(sys->step, sys->max_steps);
+    std::vector<double> h((size_t)(T + 1) * d.Bp);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(h.data(), d.partic, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b];
+        for (int64_t k = 0; k <= T; ++k) pr[(size_t)b * (T + 1) + k] = h[(size_t)k * d.Bp + sb];
     }
     return DQ_OK;
 }
@@ -1117,7 +1145,7 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
         dc.diag = a.tdiag + (size_t)(c / K) * nB;
         return dc;
     };
-    const bool obs = d.every > 0 || d.bra != nullptr;
+    const bool obs = dq::obs_on(d);
     // ---------------- forward with checkpoints ----------------
     // the last segment is taped on the way, so the reverse does not recompute it
     const int64_t last_s0 = (nck - 1) * ckpt;

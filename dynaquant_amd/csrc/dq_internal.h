@@ -85,8 +85,15 @@ struct Dev {
     double* pops;                   // [rec][g][Bp] or [rec][i][Bp] (site mode)
     const double2* bra;             // [n] or null
     double2* corr;                  // [max_steps+1][Bp]
+    double* partic;                 // [max_steps+1][Bp] participation ratio, or null
     int64_t* step;                  // device step counter
 };
+
+// any observable recorded in the prologue
+__host__ __device__ inline bool obs_on(const Dev& d)
+{
+    return d.every > 0 || d.bra != nullptr || d.partic != nullptr;
+}
 
 enum StatusBits { kStatusScaling = 1, kStatusNonfinite = 2 };
 

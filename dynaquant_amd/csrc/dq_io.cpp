@@ -188,6 +188,46 @@ dq_status dq_read_dipole_file(const char* path, int32_t n, double* mu)
     return DQ_OK;
 }
 
+// posfile, read line by line beside the dipole file (spectraClass.cu:296-360)
+dq_status dq_read_position_file(const char* path, int32_t n, double* pos)
+{
+    if (!path || !pos || n <= 0) return fail(DQ_ERR_INVALID_ARG, "null argument or n <= 0");
+    std::vector<double> v;
+    std::string err;
+    if (!read_numeric_rows(path, 3, &v, &err)) return fail(DQ_ERR_INVALID_ARG, err);
+    if ((int64_t)v.size() != 3 * (int64_t)n) return fail(DQ_ERR_INVALID_ARG, "position file: needs exactly n lines");
+    std::copy(v.begin(), v.end(), pos);
+    return DQ_OK;
+}
+
+// R x mu per site and the norms of each Cartesian component over the sites
+// (spectraClass.cu:346-411).  The reference adds the imaginary part of its
+// complex dipoles into the cross product; its dipoles are real, so that term
+// is zero and is left out here.
+dq_status dq_cd_vectors(int32_t n, const double* pos, const double* mu, double* rd, double* mu_tot,
+                        double* rd_tot)
+{
+    if (!pos || !mu || !rd || !mu_tot || !rd_tot || n <= 0) return fail(DQ_ERR_INVALID_ARG, "null argument or n <= 0");
+    double m2[3] = {0.0, 0.0, 0.0}, r2[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; ++i) {
+        const double* r = pos + 3 * (size_t)i;
+        const double* u = mu + 3 * (size_t)i;
+        double* c = rd + 3 * (size_t)i;
+        c[0] = r[1] * u[2] - r[2] * u[1];
+        c[1] = r[2] * u[0] - r[0] * u[2];
+        c[2] = r[0] * u[1] - r[1] * u[0];
+        for (int d = 0; d < 3; ++d) {
+            m2[d] += u[d] * u[d];
+            r2[d] += c[d] * c[d];
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        mu_tot[d] = std::sqrt(m2[d]);
+        rd_tot[d] = std::sqrt(r2[d]);
+    }
+    return DQ_OK;
+}
+
 // ka_paramsfile "tau sigma" (systemClass.cu:980-1071), hsr_stddevfile "sigma"
 // (systemClass.cu:910-975)
 dq_status dq_read_site_columns(const char* path, int32_t n, int32_t ncol, double* out)
@@ -312,6 +352,7 @@ dq_status dq_parse_infile(const char* path, dq_infile* out)
             else ok = false;
         } else if (key == "hamfile") copy_str(r.ham_file, word());
         else if (key == "dipolefile") copy_str(r.dipole_file, word());
+        else if (key == "posfile") copy_str(r.pos_file, word());
         else if (key == "initstatefile") copy_str(r.init_state_file, word());
         else if (key == "ka_paramsfile") copy_str(r.ka_params_file, word());
         else if (key == "hsr_stddevfile") copy_str(r.hsr_file, word());

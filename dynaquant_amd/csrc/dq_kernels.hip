@@ -208,6 +208,7 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
     int rec_idx = -1;
     if (record && d.every > 0 && step % d.every == 0 && step <= d.max_steps) rec_idx = (int)(step / d.every);
     const bool corr_on = record && d.bra != nullptr && step <= d.max_steps;
+    const bool pr_on = record && d.partic != nullptr && step <= d.max_steps;
     const bool grp = rec_idx >= 0 && !d.site_mode;
     double inv = 1.0;
     if (renorm) {
@@ -216,9 +217,9 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
     }
     const int nact = d.nact[b];
     const uint64_t tid_b = d.traj_id[b];
-    double acc[kMaxGroups + 2];
+    double acc[kMaxGroups + 3];
 #pragma unroll
-    for (int q = 0; q < kMaxGroups + 2; ++q) acc[q] = 0.0;
+    for (int q = 0; q < kMaxGroups + 3; ++q) acc[q] = 0.0;
     V2* v0 = reinterpret_cast<V2*>(d.V);
     VT* diag = reinterpret_cast<VT*>(d.diag);
     // trajectory slots are sorted by n_active (largest first), so rows at or
@@ -247,6 +248,11 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
             double2 br = d.bra[i];
             acc[kMaxGroups] += br.x * p.x + br.y * p.y;
             acc[kMaxGroups + 1] += br.x * p.y - br.y * p.x;
+        }
+        if (pr_on) {
+            // sum_i |c_i|^4 (systemClass.cu:2209-2215)
+            double pp = p.x * p.x + p.y * p.y;
+            acc[kMaxGroups + 2] += pp * pp;
         }
         if (noise && i < nact) {
             double cur = d.de[e];
@@ -292,9 +298,10 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
             v0[e] = v;
         }
     }
-    if (grp || corr_on) {
-        // quantities: group g at q = g, C(n) at q = kMaxGroups, kMaxGroups + 1
-        store_partials<PLAIN, kMaxGroups + 2>(acc, kMaxGroups + 2, d.part, d.ntile_e, t.rtile, d.Bp, b);
+    if (grp || corr_on || pr_on) {
+        // quantities: group g at q = g, C(n) at q = kMaxGroups, kMaxGroups + 1,
+        // sum |c|^4 at q = kMaxGroups + 2
+        store_partials<PLAIN, kMaxGroups + 3>(acc, kMaxGroups + 3, d.part, d.ntile_e, t.rtile, d.Bp, b);
     }
 }
 
@@ -311,7 +318,8 @@ hipError_t launch_prologue(const Dev& d, int renorm, int noise, int record, int 
     return hipGetLastError();
 }
 
-// observables of the prologue: partial sums -> pops[rec][g][b], corr[step][b]
+// observables of the prologue: partial sums -> pops[rec][g][b], corr[step][b],
+// partic[step][b]
 template <bool PLAIN>
 __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
 {
@@ -319,8 +327,9 @@ __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
     int rec_idx = -1;
     if (d.every > 0 && step % d.every == 0 && step <= d.max_steps) rec_idx = (int)(step / d.every);
     const bool corr_on = d.bra != nullptr && step <= d.max_steps;
+    const bool pr_on = d.partic != nullptr && step <= d.max_steps;
     const bool grp = rec_idx >= 0 && !d.site_mode;
-    if (!grp && !corr_on) return;
+    if (!grp && !corr_on && !pr_on) return;
     const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     if (grp) {
         for (int g = 0; g < d.ngroups; ++g) {
@@ -332,6 +341,11 @@ __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
         double re = block_tile_sum<PLAIN>(d.part, kMaxGroups, d.ntile_e, d.Bp, b);
         double im = block_tile_sum<PLAIN>(d.part, kMaxGroups + 1, d.ntile_e, d.Bp, b);
         if (threadIdx.x < (PLAIN ? 1 : 64)) d.corr[(size_t)step * d.Bp + b] = make_double2(re, im);
+    }
+    if (pr_on) {
+        // the reference stores 1 / sum |c|^4 (systemClass.cu:2251-2252)
+        double s4 = block_tile_sum<PLAIN>(d.part, kMaxGroups + 2, d.ntile_e, d.Bp, b);
+        if (threadIdx.x < (PLAIN ? 1 : 64)) d.partic[(size_t)step * d.Bp + b] = s4 > 0.0 ? 1.0 / s4 : 0.0;
     }
 }
 

@@ -16,6 +16,9 @@
  *                                            -> dq_propagate_step, dq_run
  *   popPrintClass::printOneLoopAllPops       -> dq_get_populations
  *   spectraClass::calcCorrFunc               -> dq_get_corr
+ *   spectraClass::prepForCD / prepInitState  -> dq_cd_vectors
+ *   systemClass::calcInvPatricRatio          -> dq_system_set_participation,
+ *                                               dq_get_participation
  *   spectraClass::singleSpectraCalc          -> dq_spectrum
  * Every call returns a dq_status; the message of the last failure on the
  * calling thread is returned by dq_last_error().  Nothing prints, nothing
@@ -136,6 +139,13 @@ dq_status dq_system_get_state(dq_system *sys, double *psi);
  * enables C(n) = <bra|psi(n)> every step (spectraClass.cu:509-571). */
 dq_status dq_system_set_observables(dq_system *sys, const int32_t *group, int32_t ngroups,
                                     int32_t every, int64_t max_steps, const double *bra);
+/* Participation ratio PR_b(n) = 1 / sum_i |psi_b,i(n)|^4 of the normalised
+ * state, every step n = 0..max_steps of the current observables
+ * (systemClass.cu:2209-2257, called before each step at propagate.cu:208;
+ * the reference prints the ensemble average as "invParticRat.dat",
+ * systemClass.cu:1737-1751).  Call after dq_system_set_observables, which
+ * switches it off; on = 0 switches it off. */
+dq_status dq_system_set_participation(dq_system *sys, int32_t on);
 
 /* ---- Hamiltonian generator -------------------------------------------- */
 /* Cylindrical aggregate of point dipoles: wall w has rings[w] rings of
@@ -185,6 +195,8 @@ dq_status dq_get_step(dq_system *sys, int64_t *step);
 dq_status dq_get_populations(dq_system *sys, double *pops, int64_t *nrec);
 /* corr[b][n] interleaved complex, n = 0..step */
 dq_status dq_get_corr(dq_system *sys, double *corr);
+/* pr[b][n], n = 0..step (dq_system_set_participation) */
+dq_status dq_get_participation(dq_system *sys, double *pr);
 /* sum over cycles of the clamped Expokit estimate, per trajectory */
 dq_status dq_get_error_sum(dq_system *sys, double *err);
 
@@ -200,10 +212,13 @@ dq_status dq_spmv(dq_system *sys, int32_t traj, const double *x, double *y);
  * Runs the engine's Pade kernel (padeClass.cu:204-487). */
 dq_status dq_pade_tbar(int32_t m, int32_t count, const double *alpha, const double *beta,
                        double theta, double *E, int32_t *s);
-/* Absorption spectrum from the batch-averaged C(n) (spectraClass.cu:575-750):
+/* Spectrum from the batch-averaged C(n) (spectraClass.cu:575-750):
  * window exp(-0.5((n+1)dt/sigma)^2) for n < T (sigma <= 0: none),
  * zero-pad to nfft (power of two), inverse-direction FFT on the GPU,
- * scale mu2/T.  omega[nfft] in cm^-1, spec[nfft] interleaved complex. */
+ * scale mu2/T.  mu2 is the reference's scaling constant: |mu_d|^2 for
+ * absorption, (pi/2) 1e-7 |mu_d| |R x mu_d| for circular dichroism
+ * (spectraClass.cu:641-648, dq_cd_vectors).  omega[nfft] in cm^-1,
+ * spec[nfft] interleaved complex. */
 dq_status dq_spectrum(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
                       double *omega, double *spec);
 /* In-place FFTs of `batch` interleaved complex rows of length n (power of
@@ -289,6 +304,16 @@ dq_status dq_read_hamiltonian_file(const char *path, int32_t *n_out, int64_t *nn
 dq_status dq_read_state_file(const char *path, int32_t n, double *psi);
 /* Transition dipoles (spectraClass.cu:173-262): n lines "mu_x mu_y mu_z"; mu[3n]. */
 dq_status dq_read_dipole_file(const char *path, int32_t n, double *mu);
+/* Site positions for circular dichroism, the [posfile] of infile.in
+ * (spectraClass.cu:296-360): n lines "x y z"; pos[3n]. */
+dq_status dq_read_position_file(const char *path, int32_t n, double *pos);
+/* Circular-dichroism vectors (spectraClass.cu:296-413): rd[i] = pos[i] x
+ * mu[i] per site, mu_tot[d] = |mu_d| and rd_tot[d] = |rd_d| over the sites
+ * for d = x, y, z.  Direction d of the CD run propagates psi(0) =
+ * rd_d / rd_tot[d] and records C(n) against the bra mu_d / mu_tot[d]
+ * (spectraClass.cu:463-496, 530-537).  Host code only. */
+dq_status dq_cd_vectors(int32_t n, const double *pos, const double *mu, double *rd, double *mu_tot,
+                        double *rd_tot);
 /* Per-site parameter columns: ka_paramsfile "tau sigma" (ncol 2,
  * systemClass.cu:980-1071) or hsr_stddevfile "sigma" (ncol 1,
  * systemClass.cu:910-975); out[n][ncol]. */
@@ -322,6 +347,7 @@ typedef struct dq_infile {
     int32_t init_state, init_from_file;
     char ham_file[512], dipole_file[512], init_state_file[512], ka_params_file[512], hsr_file[512],
         zz_specd_file[512], out_folder[512];
+    char pos_file[512];         /* posfile (circular dichroism) */
 } dq_infile;
 dq_status dq_parse_infile(const char *path, dq_infile *out);
 

@@ -21,6 +21,8 @@
  *   Krylov step            numpy eigh dense propagator, Rabi closed form
  *   FFT/spectrum           numpy.fft, brute DFT, single-site closed form
  *   adjoint                central finite differences of orc_run
+ *   participation ratio    closed forms (k equal weights -> k, two weights)
+ *   CD vectors (pyoracle)  numpy.cross, orthogonality to R and mu
  */
 #ifndef DQ_ORACLE_H
 #define DQ_ORACLE_H
@@ -134,23 +136,26 @@ typedef struct {
 
 /* Run one trajectory `traj`.  Outputs (any may be NULL):
  * pops[(nsteps/obs_every+1)*ngroups], corr[2*(nsteps+1)] = <bra|psi(n)>,
+ * partic[nsteps+1] = 1 / sum_i |psi_i(n)|^4 (systemClass.cu:2209-2257),
  * psi_out[2n], err_sum (sum of clamped Expokit estimates).
  * Returns 0 on success, <0 on error. */
 int orc_run(const orc_params *p, const orc_site_params *sp,
             const int32_t *rowptr, const int32_t *col, const double *val,
             const double *psi0, uint64_t traj, int64_t nsteps,
             int obs_every, const int32_t *group, int ngroups,
-            const double *bra, double *pops, double *corr,
+            const double *bra, double *pops, double *corr, double *partic,
             double *psi_out, double *err_sum);
+/* 1 / sum_i |psi_i|^4 of psi[2n] */
+double orc_participation(int n, const double *psi);
 
 /* B trajectories traj_ids[b], OpenMP over b (threads = nthreads, 0 = all).
- * pops laid out [b][obs][group], corr [b][n+1][2]. */
+ * pops laid out [b][obs][group], corr [b][n+1][2], partic [b][n+1]. */
 int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                   const int32_t *rowptr, const int32_t *col, const double *val,
                   const double *psi0, const uint64_t *traj_ids, int nb,
                   const int32_t *n_active, int64_t nsteps, int obs_every,
                   const int32_t *group, int ngroups, const double *bra,
-                  double *pops, double *corr, double *err_sum, int nthreads);
+                  double *pops, double *corr, double *partic, double *err_sum, int nthreads);
 
 /* Static site energies e0 and initial OU offsets of trajectory traj
  * (systemClass.cu:759-804, 1028-1036). */

@@ -459,11 +459,23 @@ static void record(int n, const double *psi, const int32_t *group, int ngroups, 
     }
 }
 
+/* participation ratio (systemClass.cu:2209-2257): the reference sums
+ * (re^2 + im^2)^2 over the sites and stores the reciprocal */
+double orc_participation(int n, const double *psi)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double pp = psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        s += pp * pp;
+    }
+    return 1.0 / s;
+}
+
 int orc_run(const orc_params *p, const orc_site_params *sp,
             const int32_t *rowptr, const int32_t *col, const double *val,
             const double *psi0, uint64_t traj, int64_t nsteps,
             int obs_every, const int32_t *group, int ngroups,
-            const double *bra, double *pops, double *corr,
+            const double *bra, double *pops, double *corr, double *partic,
             double *psi_out, double *err_sum)
 {
     const int n = p->n, m = p->m, nt = m + 2;
@@ -481,6 +493,7 @@ int orc_run(const orc_params *p, const orc_site_params *sp,
     int rc = 0;
     if (pops && obs_every > 0) record(n, psi, group, ngroups, pops);
     if (corr && bra) cdot(n, bra, psi, &corr[0], &corr[1]);
+    if (partic) partic[0] = orc_participation(n, psi);
     for (int64_t step = 0; step < nsteps; ++step) {
         orc_noise_step(p, sp, traj, step, de);
         for (int i = 0; i < n; ++i) diag[i] = de[i] + e0[i];
@@ -496,6 +509,7 @@ int orc_run(const orc_params *p, const orc_site_params *sp,
         if (pops && obs_every > 0 && (step + 1) % obs_every == 0)
             record(n, psi, group, ngroups, pops + (size_t)ngroups * ((step + 1) / obs_every));
         if (corr && bra) cdot(n, bra, psi, &corr[2 * (step + 1)], &corr[2 * (step + 1) + 1]);
+        if (partic) partic[step + 1] = orc_participation(n, psi);
     }
 done:
     if (psi_out) memcpy(psi_out, psi, cs * sizeof(double));
@@ -509,7 +523,7 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                   const double *psi0, const uint64_t *traj_ids, int nb,
                   const int32_t *n_active, int64_t nsteps, int obs_every,
                   const int32_t *group, int ngroups, const double *bra,
-                  double *pops, double *corr, double *err_sum, int nthreads)
+                  double *pops, double *corr, double *partic, double *err_sum, int nthreads)
 {
     const int n = p->n;
     const size_t nobs = obs_every > 0 ? (size_t)(nsteps / obs_every + 1) : 0;
@@ -540,6 +554,7 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                         obs_every, group, ngroups, bra,
                         pops ? pops + (size_t)b * nobs * ngroups : NULL,
                         corr ? corr + (size_t)b * 2 * (nsteps + 1) : NULL,
+                        partic ? partic + (size_t)b * (nsteps + 1) : NULL,
                         NULL, err_sum ? err_sum + b : NULL);
         if (r != 0) fail = 1;
         free(rp); free(cl); free(vl);

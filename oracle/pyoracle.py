@@ -89,12 +89,14 @@ def lib():
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), _ip, _ip, _dp, _dp,
                               C.c_uint64, C.c_int64, C.c_int, C.c_void_p, C.c_int,
-                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_participation.restype = C.c_double
+        L.orc_participation.argtypes = [C.c_int, _dp]
         L.orc_run_batch.restype = C.c_int
         L.orc_run_batch.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), _ip, _ip, _dp, _dp,
                                     _up, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                    C.c_int]
+                                    C.c_void_p, C.c_int]
         L.orc_fft.restype = C.c_int
         L.orc_fft.argtypes = [C.c_int, _dp, C.c_int]
         L.orc_dft.argtypes = [C.c_int, _dp, _dp, C.c_int]
@@ -309,26 +311,27 @@ def zz_oscillators(omega_cm, J, domega, temperature):
 
 
 def run(p, sites, rowptr, col, val, psi0, traj, nsteps, obs_every=0, group=None,
-        ngroups=0, bra=None):
+        ngroups=0, bra=None, partic=False):
     n = p.n
     nobs = nsteps // obs_every + 1 if obs_every > 0 else 0
     pops = np.zeros(max(nobs * ngroups, 1))
     corr = np.zeros(2 * (nsteps + 1)) if bra is not None else None
+    pr = np.zeros(nsteps + 1) if partic else None
     psi = np.zeros(2 * n)
     err = np.zeros(1)
     grp = None if group is None else _c(np.asarray(group, np.int32))
     braa = None if bra is None else c2i(bra)
     rc = lib().orc_run(C.byref(p), C.byref(sites.sp), _c(rowptr), _c(col), _c(val), c2i(psi0),
                        traj, nsteps, obs_every, _ptr(grp), ngroups, _ptr(braa),
-                       pops.ctypes.data, _ptr(corr), psi.ctypes.data, err.ctypes.data)
+                       pops.ctypes.data, _ptr(corr), _ptr(pr), psi.ctypes.data, err.ctypes.data)
     if rc != 0:
         raise RuntimeError("orc_run failed %d" % rc)
     return dict(pops=pops[: nobs * ngroups].reshape(nobs, ngroups) if nobs else None,
-                corr=None if corr is None else i2c(corr), psi=i2c(psi), err=err[0])
+                corr=None if corr is None else i2c(corr), psi=i2c(psi), err=err[0], partic=pr)
 
 
 def run_batch(p, sites, rowptr, col, val, psi0, traj_ids, nsteps, obs_every, group, ngroups,
-              n_active=None, bra=None, nthreads=0):
+              n_active=None, bra=None, nthreads=0, partic=False):
     n = p.n
     traj_ids = np.asarray(traj_ids, np.uint64)
     nb = traj_ids.size
@@ -339,17 +342,50 @@ def run_batch(p, sites, rowptr, col, val, psi0, traj_ids, nsteps, obs_every, gro
     nobs = nsteps // obs_every + 1
     pops = np.zeros(nb * nobs * ngroups)
     corr = np.zeros(nb * 2 * (nsteps + 1)) if bra is not None else None
+    pr = np.zeros(nb * (nsteps + 1)) if partic else None
     err = np.zeros(nb)
     grp = _c(np.asarray(group, np.int32))
     na = None if n_active is None else _c(np.asarray(n_active, np.int32))
     braa = None if bra is None else c2i(bra)
     rc = lib().orc_run_batch(C.byref(p), C.byref(sites.sp), _c(rowptr), _c(col), _c(val), p0,
                              traj_ids, nb, _ptr(na), nsteps, obs_every, grp.ctypes.data, ngroups,
-                             _ptr(braa), pops.ctypes.data, _ptr(corr), err.ctypes.data, nthreads)
+                             _ptr(braa), pops.ctypes.data, _ptr(corr), _ptr(pr), err.ctypes.data, nthreads)
     if rc != 0:
         raise RuntimeError("orc_run_batch failed")
     return dict(pops=pops.reshape(nb, nobs, ngroups),
-                corr=None if corr is None else i2c(corr).reshape(nb, nsteps + 1), err=err)
+                corr=None if corr is None else i2c(corr).reshape(nb, nsteps + 1), err=err,
+                partic=None if pr is None else pr.reshape(nb, nsteps + 1))
+
+
+def participation(psi):
+    """1 / sum_i |psi_i|^4 (systemClass.cu:2209-2257)"""
+    a = c2i(np.asarray(psi, np.complex128))
+    return lib().orc_participation(len(a) // 2, a)
+
+
+def cd_vectors(pos, mu):
+    """R x mu per site and the component norms (spectraClass.cu:346-411):
+    rd[i] = (y mu_z - z mu_y, z mu_x - x mu_z, x mu_y - y mu_x), mu_tot[d] =
+    sqrt(sum_i mu_d,i^2), rd_tot[d] = sqrt(sum_i rd_d,i^2), summed in site
+    order."""
+    pos = np.asarray(pos, np.float64)
+    mu = np.asarray(mu, np.float64)
+    n = len(pos)
+    rd = np.zeros((n, 3))
+    m2 = [0.0, 0.0, 0.0]
+    r2 = [0.0, 0.0, 0.0]
+    for i in range(n):
+        x, y, z = (float(v) for v in pos[i])
+        ux, uy, uz = (float(v) for v in mu[i])
+        c = (y * uz - z * uy, z * ux - x * uz, x * uy - y * ux)
+        rd[i] = c
+        for d in range(3):
+            m2[d] += (ux, uy, uz)[d] * (ux, uy, uz)[d]
+            r2[d] += c[d] * c[d]
+    return rd, np.sqrt(np.array(m2)), np.sqrt(np.array(r2))
+
+
+CD_PREFACTOR = np.pi / 2 * 1e-7     # spectraClass.cu:645-647
 
 
 def fft(x, sign=+1):
