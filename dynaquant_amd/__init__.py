@@ -32,6 +32,7 @@ EXPORTS = [
     "dq_read_hamiltonian_file", "dq_read_state_file", "dq_read_dipole_file", "dq_read_site_columns",
     "dq_read_spectral_density", "dq_parse_infile", "dq_write_pops_file", "dq_read_pops_file",
     "dq_read_position_file", "dq_cd_vectors", "dq_system_set_participation", "dq_get_participation",
+    "dq_system_set_taylor_krylov",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -202,6 +203,18 @@ class System:
             raise ValueError("omega and coeff differ in length")
         _check(lib().dq_system_set_zz_params(self._h, len(om), _p(om), _p(co)))
 
+    def set_taylor_krylov(self, p, m_aux=None, rowptr=None, col=None, val=None, diag=None):
+        """Taylor-Krylov propagation of order p with an m_aux-step Krylov part
+        on the auxiliary matrix (rowptr, col, val) + diag; p = 0: Lanczos"""
+        if p == 0:
+            _check(lib().dq_system_set_taylor_krylov(self._h, 0, 0, None, None, None, None))
+            return
+        rp = _c(rowptr, np.int32)
+        cl = _c(col, np.int32)
+        vl = _c(val, np.float64)
+        dg = _c(np.broadcast_to(np.asarray(diag, np.float64), (self.n,)), np.float64)
+        _check(lib().dq_system_set_taylor_krylov(self._h, int(p), int(m_aux), _p(rp), _p(cl), _p(vl), _p(dg)))
+
     def set_trajectories(self, traj_ids=None, n_active=None):
         t = None if traj_ids is None else _c(traj_ids, np.uint64)
         a = None if n_active is None else _c(n_active, np.int32)
@@ -367,7 +380,8 @@ class Infile(C.Structure):
         ("init_state", C.c_int32), ("init_from_file", C.c_int32),
         ("ham_file", C.c_char * 512), ("dipole_file", C.c_char * 512), ("init_state_file", C.c_char * 512),
         ("ka_params_file", C.c_char * 512), ("hsr_file", C.c_char * 512), ("zz_specd_file", C.c_char * 512),
-        ("out_folder", C.c_char * 512), ("pos_file", C.c_char * 512),
+        ("out_folder", C.c_char * 512), ("pos_file", C.c_char * 512), ("tk_mat_file", C.c_char * 512),
+        ("tk_p", C.c_int32), ("tk_m_aux", C.c_int32),
     ]
 
 

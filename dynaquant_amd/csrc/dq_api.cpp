@@ -59,6 +59,16 @@ namespace dq {
 dq_status set_error(dq_status st, const std::string& msg) { return fail(st, msg); }
 }
 
+// Row-blocked copy of a coupling matrix for the batched SpMM: per block of R
+// rows the sorted union of their columns and R values per union column (0
+// where a row lacks that column).
+struct RowBlocks {
+    int32_t* ptr = nullptr;
+    int32_t* col = nullptr;
+    float* val32 = nullptr;
+    double* val64 = nullptr;
+};
+
 struct dq_system {
     dq_config cfg{};
     Dev d{};
@@ -104,6 +114,19 @@ struct dq_system {
     int ngroups = 0, every = 0, site_mode = 0;
     int64_t max_steps = 0;
     bool have_bra = false;
+    // Taylor-Krylov propagation: order p, auxiliary Krylov m, the auxiliary
+    // matrix (host CSR for the row blocks, device CSR, row blocks, diagonal
+    // broadcast to [n][Bp] in the basis precision)
+    struct {
+        int p = 0, m = 0;
+        std::vector<int32_t> rowptr_h, col_h;
+        std::vector<double> val_h;
+        int32_t *rowptr = nullptr, *col = nullptr;
+        double* val64 = nullptr;
+        float* val32 = nullptr;
+        RowBlocks rb;
+        void* diag = nullptr;
+    } tk;
 
     ~dq_system()
     {
@@ -149,16 +172,13 @@ struct dq_system {
 
 extern "C" const char* dq_last_error(void) { return g_err.c_str(); }
 
-// Row-blocked copy of the couplings for the batched SpMM: per block of R rows
-// the sorted union of their columns and R values per union column (0 where a
-// row lacks that column), from the host CSR kept by set_hamiltonian.
-static dq_status build_row_blocks(dq_system* sys, int R)
+static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_t>& rowptr_v,
+                                 const std::vector<int32_t>& col_v, const std::vector<double>& val_v, RowBlocks* out)
 {
-    Dev& d = sys->d;
     const int n = sys->n, nblk = (n + R - 1) / R;
-    const int32_t* rowptr = sys->rowptr.data();
-    const int32_t* col = sys->col.data();
-    const double* val = sys->val.data();
+    const int32_t* rowptr = rowptr_v.data();
+    const int32_t* col = col_v.data();
+    const double* val = val_v.data();
     std::vector<int32_t> rbp(nblk + 1, 0), rbc;
     std::vector<double> rbv64;
     std::vector<int32_t> u;
@@ -181,29 +201,41 @@ static dq_status build_row_blocks(dq_system* sys, int R)
     std::vector<float> rbv32(rbv64.size());
     for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
     HIP_TRY(hipStreamSynchronize(sys->stream));
-    sys->release(*(int32_t**)&d.rb_ptr);
-    sys->release(*(int32_t**)&d.rb_col);
-    sys->release(*(float**)&d.rb_val32);
-    sys->release(*(double**)&d.rb_val64);
-    int32_t *drbp, *drbc;
-    float* drv32;
-    double* drv64;
-    DQ_TRY(sys->alloc(&drbp, rbp.size()));
-    DQ_TRY(sys->alloc(&drbc, std::max<size_t>(rbc.size(), 1)));
-    DQ_TRY(sys->alloc(&drv32, std::max<size_t>(rbv32.size(), 1)));
-    DQ_TRY(sys->alloc(&drv64, std::max<size_t>(rbv64.size(), 1)));
-    HIP_TRY(hipMemcpyAsync(drbp, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
+    sys->release(out->ptr);
+    sys->release(out->col);
+    sys->release(out->val32);
+    sys->release(out->val64);
+    DQ_TRY(sys->alloc(&out->ptr, rbp.size()));
+    DQ_TRY(sys->alloc(&out->col, std::max<size_t>(rbc.size(), 1)));
+    DQ_TRY(sys->alloc(&out->val32, std::max<size_t>(rbv32.size(), 1)));
+    DQ_TRY(sys->alloc(&out->val64, std::max<size_t>(rbv64.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(out->ptr, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
     if (!rbc.empty()) {
-        HIP_TRY(hipMemcpyAsync(drbc, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(drv32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(drv64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(out->col, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(out->val32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(out->val64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
     }
     HIP_TRY(hipStreamSynchronize(sys->stream));
-    d.rb_ptr = drbp; d.rb_col = drbc; d.rb_val32 = drv32; d.rb_val64 = drv64;
+    return DQ_OK;
+}
+
+// row blocks of R rows for the Hamiltonian (from the host CSR kept by
+// set_hamiltonian) and, with Taylor-Krylov on, for the auxiliary matrix
+static dq_status build_row_blocks(dq_system* sys, int R)
+{
+    Dev& d = sys->d;
+    RowBlocks a;
+    a.ptr = const_cast<int32_t*>(d.rb_ptr);
+    a.col = const_cast<int32_t*>(d.rb_col);
+    a.val32 = const_cast<float*>(d.rb_val32);
+    a.val64 = const_cast<double*>(d.rb_val64);
+    DQ_TRY(make_row_blocks(sys, R, sys->rowptr, sys->col, sys->val, &a));
+    d.rb_ptr = a.ptr; d.rb_col = a.col; d.rb_val32 = a.val32; d.rb_val64 = a.val64;
+    if (sys->tk.p > 0) DQ_TRY(make_row_blocks(sys, R, sys->tk.rowptr_h, sys->tk.col_h, sys->tk.val_h, &sys->tk.rb));
     d.rb = R;
     if (!d.plain) {
         d.rt_s = 4 * R;
-        d.ntile_s = (n + d.rt_s - 1) / d.rt_s;
+        d.ntile_s = (sys->n + d.rt_s - 1) / d.rt_s;
     }
     return DQ_OK;
 }
@@ -252,6 +284,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.plain = plain;
     d.rt_e = plain ? 1024 : 128;
     d.rb = 4;
+    d.aug = 2;
     d.spmm_tpl = (d.mixed && d.Bp % 128 == 0) ? 2 : 1;
     d.spmm_phases = 1;
     d.spmm_pipe = 0;
@@ -398,6 +431,124 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
     return DQ_OK;
 }
 
+static void release_tk(dq_system* sys)
+{
+    Dev& d = sys->d;
+    sys->release(sys->tk.rowptr);
+    sys->release(sys->tk.col);
+    sys->release(sys->tk.val64);
+    sys->release(sys->tk.val32);
+    sys->release(sys->tk.rb.ptr);
+    sys->release(sys->tk.rb.col);
+    sys->release(sys->tk.rb.val32);
+    sys->release(sys->tk.rb.val64);
+    sys->release(sys->tk.diag);
+    sys->release(d.tk_t);
+    sys->release(d.tk_u);
+    sys->release(*(double2**)&d.tk_coef);
+    sys->release(d.tk_nrm);
+    sys->release(d.tk_err);
+    sys->tk.p = sys->tk.m = 0;
+    d.tk_p = 0;
+}
+
+// taylorKrylovClass.cu:133-215 (set-up), the tkmatfile / tk_p / tk_m_aux keys
+// (systemClass.cu:455-469)
+dq_status dq_system_set_taylor_krylov(dq_system* sys, int32_t p, int32_t m_aux, const int32_t* rowptr,
+                                      const int32_t* col, const double* val, const double* diag)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    Dev& d = sys->d;
+    const int n = sys->n;
+    if (p != 0) {
+        if (p < 2 || m_aux < 2 || m_aux > sys->m || m_aux + p > dq::kMaxM + 2)
+            return fail(DQ_ERR_INVALID_ARG, "Taylor-Krylov needs p >= 2, 2 <= m_aux <= m and m_aux + p <= 34");
+        if (!rowptr || !col || !val || !diag) return fail(DQ_ERR_INVALID_ARG, "null auxiliary matrix");
+        if (rowptr[0] != 0) return fail(DQ_ERR_INVALID_ARG, "rowptr[0] != 0");
+        for (int i = 0; i < n; ++i) {
+            if (rowptr[i + 1] < rowptr[i]) return fail(DQ_ERR_INVALID_ARG, "rowptr not monotone");
+            if (!std::isfinite(diag[i])) return fail(DQ_ERR_NONFINITE, "non-finite auxiliary diagonal");
+            for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+                if (col[q] < 0 || col[q] >= n) return fail(DQ_ERR_INVALID_ARG, "column out of range");
+                if (col[q] == i) return fail(DQ_ERR_INVALID_ARG, "diagonal entries belong in diag");
+                if (!std::isfinite(val[q])) return fail(DQ_ERR_NONFINITE, "non-finite auxiliary coupling");
+            }
+        }
+    }
+    sys->drop_graph();
+    hipStreamSynchronize(sys->stream);
+    release_tk(sys);
+    if (p == 0) return DQ_OK;
+    const int64_t nnz = rowptr[n];
+    const size_t nB = (size_t)n * d.Bp;
+    sys->tk.rowptr_h.assign(rowptr, rowptr + n + 1);
+    sys->tk.col_h.assign(col, col + nnz);
+    sys->tk.val_h.assign(val, val + nnz);
+    DQ_TRY(sys->alloc(&sys->tk.rowptr, (size_t)n + 1));
+    DQ_TRY(sys->alloc(&sys->tk.col, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(sys->alloc(&sys->tk.val64, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(sys->alloc(&sys->tk.val32, (size_t)std::max<int64_t>(nnz, 1)));
+    std::vector<float> f32(nnz);
+    for (int64_t q = 0; q < nnz; ++q) f32[q] = (float)val[q];
+    HIP_TRY(hipMemcpyAsync(sys->tk.rowptr, rowptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, sys->stream));
+    if (nnz > 0) {
+        HIP_TRY(hipMemcpyAsync(sys->tk.col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(sys->tk.val64, val, sizeof(double) * nnz, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(sys->tk.val32, f32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, sys->stream));
+    }
+    // the auxiliary diagonal for every trajectory slot, in the basis precision
+    if (d.mixed) {
+        float* dg;
+        DQ_TRY(sys->alloc(&dg, nB));
+        std::vector<float> h(nB);
+        for (int i = 0; i < n; ++i)
+            for (int b = 0; b < d.Bp; ++b) h[(size_t)i * d.Bp + b] = (float)diag[i];
+        HIP_TRY(hipMemcpyAsync(dg, h.data(), sizeof(float) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        sys->tk.diag = dg;
+    } else {
+        double* dg;
+        DQ_TRY(sys->alloc(&dg, nB));
+        std::vector<double> h(nB);
+        for (int i = 0; i < n; ++i)
+            for (int b = 0; b < d.Bp; ++b) h[(size_t)i * d.Bp + b] = diag[i];
+        HIP_TRY(hipMemcpyAsync(dg, h.data(), sizeof(double) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        sys->tk.diag = dg;
+    }
+    // c^k / k! of c = -i tau / hbar by repeated complex products from 1
+    // (taylorKrylovClass.cu:200-214, cppUtility.cpp:52-79); k! in double,
+    // where the reference's int factorial would overflow past 12!
+    std::vector<double2> coef(p);
+    {
+        const double cr = 0.0, ci = -d.theta;
+        double zr = 1.0, zi = 0.0, fact = 1.0;
+        for (int k = 0; k < p; ++k) {
+            if (k > 0) {
+                const double r = zr * cr - zi * ci, im = zr * ci + zi * cr;
+                zr = r;
+                zi = im;
+                fact *= (double)k;
+            }
+            coef[k] = make_double2(zr / fact, zi / fact);
+        }
+    }
+    double2* dc;
+    DQ_TRY(sys->alloc(&dc, (size_t)p));
+    HIP_TRY(hipMemcpyAsync(dc, coef.data(), sizeof(double2) * p, hipMemcpyHostToDevice, sys->stream));
+    DQ_TRY(sys->alloc(&d.tk_t, nB));
+    DQ_TRY(sys->alloc(&d.tk_u, 2 * nB));
+    DQ_TRY(sys->alloc(&d.tk_nrm, (size_t)d.Bp));
+    DQ_TRY(sys->alloc(&d.tk_err, (size_t)d.Bp));
+    d.tk_coef = dc;
+    sys->tk.p = p;
+    sys->tk.m = m_aux;
+    DQ_TRY(make_row_blocks(sys, d.rb, sys->tk.rowptr_h, sys->tk.col_h, sys->tk.val_h, &sys->tk.rb));
+    d.tk_p = p;
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    return DQ_OK;
+}
+
 dq_status dq_system_set_ou_params(dq_system* sys, const double* tau_fs, const double* sigma)
 {
     if (!sys || !tau_fs || !sigma) return fail(DQ_ERR_INVALID_ARG, "null argument");
@@ -528,6 +679,10 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
     const size_t vb = d.mixed ? sizeof(float2) : sizeof(double2);
     HIP_TRY(hipMemsetAsync(d.V, 0, vb * nB * (sys->m + 1), sys->stream));
     HIP_TRY(hipMemsetAsync(d.psi, 0, sizeof(double2) * nB, sys->stream));
+    if (d.tk_t) {
+        HIP_TRY(hipMemsetAsync(d.tk_t, 0, sizeof(double2) * nB, sys->stream));
+        HIP_TRY(hipMemsetAsync(d.tk_u, 0, sizeof(double2) * nB * 2, sys->stream));
+    }
     HIP_TRY(hipStreamSynchronize(sys->stream));
     sys->have_disorder = false;
     sys->have_state = false;
@@ -698,12 +853,10 @@ static hipError_t collect_timings(dq_system* sys)
 
 extern "C" {
 
-static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
-                                    bool obs, hipStream_t st, bool advance)
+// Lanczos decomposition of d's matrix on basis slot 0, then the Pade kernel
+static hipError_t enqueue_lanczos_pade(dq_system* sys, const Dev& d, hipStream_t st)
 {
     hipError_t e;
-    if ((e = timed(sys, DQ_K_PROLOGUE, [&] { return dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, 1, st); })) != hipSuccess) return e;
-    if (first_in_step && obs && (e = timed(sys, DQ_K_NORM, [&] { return dq::launch_reduce_obs(d, st); })) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_spmm(d, j, 0, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_SCALARS, [&] { return dq::launch_scalars_h(d, j, st); })) != hipSuccess) return e;
@@ -711,9 +864,51 @@ static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_
         if ((e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
     }
     if ((e = timed(sys, DQ_K_SPMM_LAST, [&] { return dq::launch_spmm(d, d.m, 1, st); })) != hipSuccess) return e;
-    if ((e = timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); })) != hipSuccess) return e;
-    if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_combine(d, st); })) != hipSuccess) return e;
-    return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, (last_in_step && advance) ? 1 : 0, st); });
+    return timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); });
+}
+
+// the auxiliary Krylov part of a Taylor-Krylov cycle: d with the auxiliary
+// matrix, its m and augmentation p
+static Dev tk_dev(const dq_system* sys, const Dev& d)
+{
+    Dev a = d;
+    a.m = sys->tk.m;
+    a.aug = sys->tk.p;
+    a.tk = 1;
+    a.rowptr = sys->tk.rowptr;
+    a.col = sys->tk.col;
+    a.val64 = sys->tk.val64;
+    a.val32 = sys->tk.val32;
+    a.rb_ptr = sys->tk.rb.ptr;
+    a.rb_col = sys->tk.rb.col;
+    a.rb_val32 = sys->tk.rb.val32;
+    a.rb_val64 = sys->tk.rb.val64;
+    a.diag = sys->tk.diag;
+    a.g = 1.0;
+    return a;
+}
+
+static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
+                                    bool obs, hipStream_t st, bool advance)
+{
+    hipError_t e;
+    const bool tk = sys->tk.p > 0;
+    if ((e = timed(sys, DQ_K_PROLOGUE, [&] { return dq::launch_prologue(d, 1, first_in_step, first_in_step && obs, tk ? 0 : 1, st); })) != hipSuccess) return e;
+    if (first_in_step && obs && (e = timed(sys, DQ_K_NORM, [&] { return dq::launch_reduce_obs(d, st); })) != hipSuccess) return e;
+    if (!tk) {
+        if ((e = enqueue_lanczos_pade(sys, d, st)) != hipSuccess) return e;
+        if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_combine(d, st); })) != hipSuccess) return e;
+        return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(d, (last_in_step && advance) ? 1 : 0, st); });
+    }
+    // Taylor-Krylov (taylorKrylovClass.cu:217-434): Taylor part with the
+    // Hamiltonian, Krylov part with the auxiliary matrix
+    for (int k = 1; k <= sys->tk.p; ++k)
+        if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_tk_power(d, k, st); })) != hipSuccess) return e;
+    if ((e = timed(sys, DQ_K_NORM, [&] { return dq::launch_tk_start(d, st); })) != hipSuccess) return e;
+    const Dev a = tk_dev(sys, d);
+    if ((e = enqueue_lanczos_pade(sys, a, st)) != hipSuccess) return e;
+    if ((e = timed(sys, DQ_K_COMBINE, [&] { return dq::launch_tk_combine(a, st); })) != hipSuccess) return e;
+    return timed(sys, DQ_K_NORM, [&] { return dq::launch_finish_norm(a, (last_in_step && advance) ? 1 : 0, st); });
 }
 
 static hipError_t enqueue_cycle_dev(dq_system* sys, const Dev& d, bool first_in_step, bool last_in_step,
@@ -1080,6 +1275,7 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     DQ_TRY(check_ready(sys));
     Dev& d = sys->d;
     if (d.plain || d.mixed) return fail(DQ_ERR_INVALID_ARG, "the adjoint runs on batched FP64 systems (force_batched)");
+    if (sys->tk.p > 0) return fail(DQ_ERR_INVALID_ARG, "the adjoint differentiates the Lanczos propagator, not Taylor-Krylov");
     if (nsteps < 1 || ckpt < 1) return fail(DQ_ERR_INVALID_ARG, "nsteps and ckpt must be >= 1");
     if (sys->step != 0) return fail(DQ_ERR_STATE, "the adjoint starts from step 0 (reset_disorder and set_state)");
     if (!loss || !grad_e || !grad_g) return fail(DQ_ERR_INVALID_ARG, "null output");

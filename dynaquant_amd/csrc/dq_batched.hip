@@ -282,6 +282,99 @@ hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
+// Taylor part of a Taylor-Krylov cycle (taylorKrylovClass.cu:226-322), batched:
+// u_k = A u_{k-1} with the row-blocked couplings of the SpMM above, FP64
+// vectors (|A^k psi| grows as |A|^k), u_0 = psi; t = psi + c u_1 at k = 1,
+// t += c^k u_k / k! for 1 < k < p; at k = p the partial |u_p|^2 per SpMM tile.
+// ---------------------------------------------------------------------------
+template <int kRB>
+__global__ __launch_bounds__(kThreads) void k_tk_power_rb(Dev d, int k)
+{
+    const int ntt = d.sn / 64;
+    const int logical = xcd_tile(blockIdx.x, gridDim.x);
+    if (logical >= d.ntile_s * ntt) return;
+    const int rtile = logical / ntt, ttile = d.s0 / 64 + logical % ntt;
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int Bp = d.Bp;
+    const int b = ttile * 64 + lane;
+    const size_t nB = (size_t)d.n * Bp;
+    const double2* __restrict__ x = k == 1 ? d.psi : d.tk_u + (size_t)((k - 1) & 1) * nB;
+    double2* __restrict__ y = d.tk_u + (size_t)(k & 1) * nB;
+    const double2* __restrict__ xs = x + (size_t)ttile * 64;
+    const double* __restrict__ rv = d.rb_val64;
+    const int32_t* __restrict__ rc = d.rb_col;
+    const bool last = k == d.tk_p;
+    const double2 ck = last ? make_double2(0.0, 0.0) : d.tk_coef[k];
+    const int nblk = (d.n + kRB - 1) / kRB;
+    const int rb = rtile * 4 + wave;
+    double acc = 0.0;
+    const int slice_nact = uniform(d.nact[ttile * 64]);
+    if (rb < nblk && rb * kRB < slice_nact) {
+        const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+        double ar[1][kRB], ai[1][kRB];
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) { ar[0][r] = 0.0; ai[0][r] = 0.0; }
+        int p = u0;
+        for (; p + 8 <= u1; p += 8) {
+            double xr[8][1], xi[8][1];
+            gather8<double, 1>(xs, rc, p, Bp, lane, xr, xi);
+            fma8<double, 1, kRB>(rv, p, xr, xi, ar, ai);
+        }
+        for (; p < u1; ++p) {
+            const double2 q = xs[(size_t)uniform(rc[p]) * Bp + lane];
+#pragma unroll
+            for (int r = 0; r < kRB; ++r) {
+                const double v = rv[(size_t)p * kRB + r];
+                ar[0][r] = fmav(v, q.x, ar[0][r]);
+                ai[0][r] = fmav(v, q.y, ai[0][r]);
+            }
+        }
+        const int nact = d.nact[b];
+#pragma unroll
+        for (int r = 0; r < kRB; ++r) {
+            const int i = rb * kRB + r;
+            if (i < d.n) {
+                const size_t e = (size_t)i * Bp + b;
+                const double2 xv = x[e];
+                const double di = d.mixed ? (double)reinterpret_cast<const float*>(d.diag)[e]
+                                          : reinterpret_cast<const double*>(d.diag)[e];
+                double wr = fmav(d.g, ar[0][r], di * xv.x);
+                double wi = fmav(d.g, ai[0][r], di * xv.y);
+                if (i >= nact) { wr = 0.0; wi = 0.0; }
+                y[e] = make_double2(wr, wi);
+                if (!last) {
+                    double2 tv = k == 1 ? xv : d.tk_t[e];
+                    tv.x += ck.x * wr - ck.y * wi;
+                    tv.y += ck.x * wi + ck.y * wr;
+                    d.tk_t[e] = tv;
+                } else {
+                    acc += wr * wr + wi * wi;
+                }
+            }
+        }
+    }
+    if (last) {
+        __shared__ double lds[4][64];
+        lds[wave][lane] = acc;
+        __syncthreads();
+        if (wave == 0)
+            d.part[(size_t)rtile * Bp + b] = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
+    }
+}
+
+hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s)
+{
+    const dim3 grid(bpad8(d.ntile_s * (d.sn / 64))), block(kThreads);
+    switch (d.rb) {
+    case 2: hipLaunchKernelGGL((k_tk_power_rb<2>), grid, block, 0, s, d, k); break;
+    case 8: hipLaunchKernelGGL((k_tk_power_rb<8>), grid, block, 0, s, d, k); break;
+    default: hipLaunchKernelGGL((k_tk_power_rb<4>), grid, block, 0, s, d, k); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/4) or 1).
 // Every block sums <v_j|w> over the SpMM tiles and rebuilds s_j and the Gram
 // row for its 64 trajectories; block y == 0 stores them; wave w of block y
@@ -545,11 +638,14 @@ hipError_t launch_update_b(const Dev& d, int j, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // Combine psi = y_0 psi + sum_{k>=1} (y_k s_k) v_k (propStep.cu:31-44) and the
-// partial |psi|^2; same 16 x 4 lane layout as the update.
+// partial |psi|^2; same 16 x 4 lane layout as the update.  TK: the Krylov part
+// of a Taylor-Krylov cycle, psi = t + sum_{k>=0} (y_k s_k) v_k
+// (taylorKrylovClass.cu:372-418).
 // ---------------------------------------------------------------------------
-template <typename VT, int KW>
+template <typename VT, int KW, bool TK>
 __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 {
+    constexpr int k0 = TK ? 0 : 1;
     using V2 = typename BVec2<VT>::T;
     __shared__ double2 s_y[4 * KW][16];
     __shared__ double lds[4][64];
@@ -585,9 +681,9 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k >= 1 && k < m) nk[u] = V[(size_t)k * nB + e];
+            if (k >= k0 && k < m) nk[u] = V[(size_t)k * nB + e];
         }
-        if (kg == 0) np = d.psi[e];
+        if (kg == 0) np = TK ? d.tk_t[e] : d.psi[e];
     }
     for (int i = r0 + wave; i < r1; i += 4) {
         const size_t e = (size_t)i * Bp + b;
@@ -600,19 +696,24 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
             for (int u = 0; u < KW; ++u) {
                 const int k = kg + 4 * u;
-                if (k >= 1 && k < m) nk[u] = V[(size_t)k * nB + en];
+                if (k >= k0 && k < m) nk[u] = V[(size_t)k * nB + en];
             }
-            if (kg == 0) np = d.psi[en];
+            if (kg == 0) np = TK ? d.tk_t[en] : d.psi[en];
         }
         double pr = 0.0, pi = 0.0;
         if (kg == 0) {
-            pr = y[0].x * p.x - y[0].y * p.y;
-            pi = y[0].x * p.y + y[0].y * p.x;
+            if constexpr (TK) {
+                pr = p.x;
+                pi = p.y;
+            } else {
+                pr = y[0].x * p.x - y[0].y * p.y;
+                pi = y[0].x * p.y + y[0].y * p.x;
+            }
         }
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k >= 1 && k < m) {
+            if (k >= k0 && k < m) {
                 const double xr = (double)vk[u].x, xi = (double)vk[u].y;
                 pr += y[u].x * xr - y[u].y * xi;
                 pi += y[u].x * xi + y[u].y * xr;
@@ -633,20 +734,27 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
         d.part[(size_t)rtile * Bp + b] = (lds[0][lane] + lds[1][lane]) + (lds[2][lane] + lds[3][lane]);
 }
 
-hipError_t launch_combine_b(const Dev& d, hipStream_t s)
+template <bool TK>
+static void launch_combine_bt(const Dev& d, hipStream_t s)
 {
     dim3 grid(bpad8(d.ntile_e * (d.sn / 16))), block(kThreads);
     if (d.mixed) {
-        if (d.m <= 8) hipLaunchKernelGGL((k_combine_b<float, 2>), grid, block, 0, s, d);
-        else if (d.m <= 16) hipLaunchKernelGGL((k_combine_b<float, 4>), grid, block, 0, s, d);
-        else if (d.m <= 24) hipLaunchKernelGGL((k_combine_b<float, 6>), grid, block, 0, s, d);
-        else hipLaunchKernelGGL((k_combine_b<float, 8>), grid, block, 0, s, d);
+        if (d.m <= 8) hipLaunchKernelGGL((k_combine_b<float, 2, TK>), grid, block, 0, s, d);
+        else if (d.m <= 16) hipLaunchKernelGGL((k_combine_b<float, 4, TK>), grid, block, 0, s, d);
+        else if (d.m <= 24) hipLaunchKernelGGL((k_combine_b<float, 6, TK>), grid, block, 0, s, d);
+        else hipLaunchKernelGGL((k_combine_b<float, 8, TK>), grid, block, 0, s, d);
     } else {
-        if (d.m <= 8) hipLaunchKernelGGL((k_combine_b<double, 2>), grid, block, 0, s, d);
-        else if (d.m <= 16) hipLaunchKernelGGL((k_combine_b<double, 4>), grid, block, 0, s, d);
-        else if (d.m <= 24) hipLaunchKernelGGL((k_combine_b<double, 6>), grid, block, 0, s, d);
-        else hipLaunchKernelGGL((k_combine_b<double, 8>), grid, block, 0, s, d);
+        if (d.m <= 8) hipLaunchKernelGGL((k_combine_b<double, 2, TK>), grid, block, 0, s, d);
+        else if (d.m <= 16) hipLaunchKernelGGL((k_combine_b<double, 4, TK>), grid, block, 0, s, d);
+        else if (d.m <= 24) hipLaunchKernelGGL((k_combine_b<double, 6, TK>), grid, block, 0, s, d);
+        else hipLaunchKernelGGL((k_combine_b<double, 8, TK>), grid, block, 0, s, d);
     }
+}
+
+hipError_t launch_combine_b(const Dev& d, hipStream_t s)
+{
+    if (d.tk) launch_combine_bt<true>(d, s);
+    else launch_combine_bt<false>(d, s);
     return hipGetLastError();
 }
 

@@ -86,6 +86,15 @@ struct Dev {
     const double2* bra;             // [n] or null
     double2* corr;                  // [max_steps+1][Bp]
     double* partic;                 // [max_steps+1][Bp] participation ratio, or null
+    // Taylor-Krylov propagation (taylorKrylovClass.cu:133-434)
+    int aug;                        // augmentation of Tbar: 2, or p for the auxiliary Krylov part
+    int tk;                         // 1: this Dev is the auxiliary Krylov part of a Taylor-Krylov cycle
+    int tk_p;                       // Taylor order p (0: Lanczos propagation)
+    double2* tk_t;                  // [n][Bp] Taylor part sum_{k<p} c^k A^k psi / k!
+    double2* tk_u;                  // 2 x [n][Bp] A^k psi, alternating
+    const double2* tk_coef;         // [p] c^k / k!
+    double* tk_nrm;                 // [Bp] |A^p psi|
+    double* tk_err;                 // [Bp] Expokit estimate of the auxiliary part (unclamped)
     int64_t* step;                  // device step counter
 };
 
@@ -122,6 +131,11 @@ hipError_t launch_advance_step(const Dev& d, hipStream_t s);
 hipError_t launch_pade_tbar(int m, int count, const double* alpha, const double* beta,
                             double theta, double2* E, int* s_out, hipStream_t s);
 hipError_t launch_fft(int n, int batch, double2* a, double2* b, int sign, hipStream_t s);
+// Taylor-Krylov (dq_taylor.hip)
+hipError_t launch_tk_power(const Dev& d, int k, hipStream_t s);
+hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s);
+hipError_t launch_tk_start(const Dev& d, hipStream_t s);
+hipError_t launch_tk_combine(const Dev& d, hipStream_t s);
 hipError_t launch_spectrum_prep(const Dev& d, int64_t T, double dt, double sigma, int nfft,
                                 double2* buf, hipStream_t s);
 

@@ -286,6 +286,8 @@ dq_status dq_parse_infile(const char* path, dq_infile* out)
     r.dt = 1.0;
     r.m = 12;
     r.ka_substeps = 1;
+    r.tk_p = 10;                // systemClass.cu:70-71
+    r.tk_m_aux = 24;
     r.steps_per_loop = 0;
     r.noise_model = DQ_NOISE_NONE;
     r.gauss_width = 1.0;
@@ -348,11 +350,14 @@ dq_status dq_parse_infile(const char* path, dq_infile* out)
         } else if (key == "propmode") {
             const std::string w = word();
             if (w == "PROP_MODE_LANCZOS") r.prop_mode = 0;
-            else if (w == "PROP_MODE_TAYLOR_KRYLOV") r.prop_mode = 1;
+            else if (w == "PROP_MODE_TAYLOR_LANCZOS") r.prop_mode = 1;
             else ok = false;
         } else if (key == "hamfile") copy_str(r.ham_file, word());
         else if (key == "dipolefile") copy_str(r.dipole_file, word());
         else if (key == "posfile") copy_str(r.pos_file, word());
+        else if (key == "tkmatfile") copy_str(r.tk_mat_file, word());
+        else if (key == "tk_p") r.tk_p = std::atoi(v.c_str());
+        else if (key == "tk_m_aux") r.tk_m_aux = std::atoi(v.c_str());
         else if (key == "initstatefile") copy_str(r.init_state_file, word());
         else if (key == "ka_paramsfile") copy_str(r.ka_params_file, word());
         else if (key == "hsr_stddevfile") copy_str(r.hsr_file, word());

@@ -626,7 +626,10 @@ hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s)
 constexpr int kNMax = kMaxM + 2;
 constexpr int kNN = kNMax * kNMax;
 
-__device__ __forceinline__ double tbar_at(int i, int jj, int m, const double* al, const double* be)
+// augmented tridiagonal of size m + aug (gpuLanczosHelper.cu:14-60): T, a 1
+// at (0, m) and ones on the superdiagonal of the trailing aug x aug block;
+// column m + j - 1 of its exponential holds the phi_j block
+__device__ __forceinline__ double tbar_at(int i, int jj, int m, int aug, const double* al, const double* be)
 {
     if (i < m && jj < m) {
         if (i == jj) return al[i];
@@ -635,7 +638,7 @@ __device__ __forceinline__ double tbar_at(int i, int jj, int m, const double* al
         return 0.0;
     }
     if (i == 0 && jj == m) return 1.0;
-    if (i == m && jj == m + 1) return 1.0;
+    if (i >= m && jj == i + 1 && jj < m + aug) return 1.0;
     return 0.0;
 }
 
@@ -666,10 +669,10 @@ __device__ void mm_cplx(int n, const double2* A, const double2* B, double2* C)
 }
 
 // Returns the scaling exponent s (or -1 when s >= 50); result in *Eout.
-__device__ int pade_block(int m, const double* al, const double* be, double theta,
+__device__ int pade_block(int m, int aug, const double* al, const double* be, double theta,
                           double* sm, const double2** Eout)
 {
-    const int n = m + 2, nn = n * n;
+    const int n = m + aug, nn = n * n;
     double2* X0 = reinterpret_cast<double2*>(sm);
     double2* X1 = reinterpret_cast<double2*>(sm + 2 * kNN);
     double* R0 = sm + 4 * kNN;
@@ -685,7 +688,7 @@ __device__ int pade_block(int m, const double* al, const double* be, double thet
         double nrm = 0.0;
         for (int i = 0; i < n; ++i) {
             double rs = 0.0;
-            for (int jj = 0; jj < n; ++jj) rs += __builtin_fabs(cim * tbar_at(i, jj, m, al, be));
+            for (int jj = 0; jj < n; ++jj) rs += __builtin_fabs(cim * tbar_at(i, jj, m, aug, al, be));
             if (rs > nrm) nrm = rs;
         }
         int s = 0;
@@ -702,7 +705,7 @@ __device__ int pade_block(int m, const double* al, const double* be, double thet
     for (int k = 1; k <= 6; ++k) cc[k] = cc[k - 1] * (6.0 + 1. - k) / ((2 * 6.0 + 1. - k) * k);
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {
         const int i = e / n, jj = e % n;
-        R0[e] = (cim * tbar_at(i, jj, m, al, be)) * sc;      // a
+        R0[e] = (cim * tbar_at(i, jj, m, aug, al, be)) * sc;      // a
     }
     __syncthreads();
     mm_real(n, R0, R0, R1);            // P2
@@ -844,28 +847,37 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
         return;
     }
     const double2* E;
-    const int s = pade_block(m, al, be, d.theta, sm, &E);
+    const int s = pade_block(m, d.aug, al, be, d.theta, sm, &E);
     if (s < 0) {
         if (threadIdx.x == 0) atomicOr(d.status, (int)kStatusScaling);
         for (int k = threadIdx.x; k < m; k += kThreads) d.ycoef[(size_t)k * Bp + b] = make_double2(0.0, 0.0);
         return;
     }
-    const int n = m + 2;
+    const int n = m + d.aug;
+    // Lanczos: column 0 (propStep.cu:31-44).  Taylor-Krylov: |A^p psi| times
+    // the rows 0..m-1 of column m + p - 1 (taylorKrylovClass.cu:372-397)
+    const int ecol = d.tk ? n - 1 : 0;
+    const double f = d.tk ? d.tk_nrm[b] : 1.0;
     for (int k = threadIdx.x; k < m; k += kThreads) {
-        const double2 y = E[k * n + 0];
-        const double sk = d.sc[(size_t)k * Bp + b];
+        const double2 y = E[k * n + ecol];
+        const double sk = d.sc[(size_t)k * Bp + b] * f;
         d.ycoef[(size_t)k * Bp + b] = make_double2(y.x * sk, y.y * sk);
     }
     if (threadIdx.x == 0) {
-        // Expokit estimate (padeClass.cu:749-782), clamped (systemClass.cu:2075-2077)
+        // Expokit estimate (padeClass.cu:749-782), clamped (systemClass.cu:2075-2077);
+        // the Taylor-Krylov cycle adds |1 - |psi|| before the clamp (k_finish_norm)
         const double hm = __builtin_fabs(be[m - 1]);
         const double2 e1 = E[(m - 1) * n + m], e2 = E[(m - 1) * n + m + 1];
         const double err1 = hm * __builtin_sqrt(e1.x * e1.x + e1.y * e1.y);
         const double err2 = hm * s_hq * __builtin_sqrt(e2.x * e2.x + e2.y * e2.y);
         double err = err1 > err2 ? err2 / (1 - err2 / err1) : err1;
-        if (err < 1.110223e-16) err = 1.110223e-16;
         if (!(err == err)) atomicOr(d.status, (int)kStatusNonfinite);
-        d.err_sum[b] += err;
+        if (d.tk) {
+            d.tk_err[b] = err;
+        } else {
+            if (err < 1.110223e-16) err = 1.110223e-16;
+            d.err_sum[b] += err;
+        }
     }
 }
 
@@ -889,7 +901,7 @@ __global__ __launch_bounds__(kThreads) void k_pade_tbar(int m, const double* alp
     }
     __syncthreads();
     const double2* E;
-    const int s = pade_block(m, al, be, theta, sm, &E);
+    const int s = pade_block(m, 2, al, be, theta, sm, &E);
     if (threadIdx.x == 0) s_out[c] = s;
     const int n = m + 2;
     if (s < 0) return;
@@ -977,7 +989,16 @@ __global__ __launch_bounds__(kThreads) void k_finish_norm(Dev d, int advance_ste
 {
     const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     const double s = block_tile_sum<PLAIN>(d.part, 0, d.ntile_e, d.Bp, b);
-    if (threadIdx.x < (PLAIN ? 1u : 64u)) d.nrm2[b] = s;
+    if (threadIdx.x < (PLAIN ? 1u : 64u)) {
+        d.nrm2[b] = s;
+        if (d.tk && b < d.B) {
+            // getTaylorKrylovError (taylorKrylovClass.cu:426-440): Krylov
+            // estimate + |1 - |psi||, clamped (systemClass.cu:2075-2077)
+            double err = d.tk_err[b] + __builtin_fabs(1. - __builtin_sqrt(s));
+            if (err < 1.110223e-16) err = 1.110223e-16;
+            d.err_sum[b] += err;
+        }
+    }
     if (advance_step && blockIdx.x == 0 && threadIdx.x == 0) *d.step = *d.step + 1;
 }
 
@@ -1004,6 +1025,192 @@ __global__ void k_set_unit_norm(Dev d)
 hipError_t launch_set_unit_norm(const Dev& d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_unit_norm, dim3(1), dim3(kThreads), 0, s, d);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Taylor-Krylov propagation (taylorKrylovClass.cu:217-434):
+//   psi' = sum_{k<p} c^k A^k psi / k!  +  |A^p psi| Q_m [exp(c Tbar_p)]_{0:m, m+p-1}
+// c = -i tau / hbar, A the noisy Hamiltonian of the trajectory, Q_m and
+// Tbar_p (augmentation p) from the Lanczos decomposition of the constant
+// auxiliary matrix on A^p psi / |A^p psi|.
+//   tk_power(k), k = 1..p   u_k = A u_{k-1} (u_0 = psi); t += c^k u_k / k!
+//                           for k < p (t = psi + c u_1 at k = 1); partial
+//                           |u_p|^2 at k = p (batched: row-blocked, in
+//                           dq_batched.hip)
+//   tk_start                |u_p| per trajectory, basis slot 0 = u_p / |u_p|
+//   Lanczos + pade on the auxiliary matrix (Dev.tk = 1, Dev.aug = p)
+//   tk_combine              psi = t + sum_{k<m} y_k s_k v_k, partial |psi|^2
+// The Taylor vectors are FP64 in both precisions: |A^k psi| grows as |A|^k.
+// ---------------------------------------------------------------------------
+
+// single trajectory: 16 lanes per row, as k_spmv_plain
+template <typename VT>
+__global__ __launch_bounds__(kThreads) void k_tk_power_plain(Dev d, int k)
+{
+    const int rtile = xcd_tile(blockIdx.x, gridDim.x);
+    if (rtile >= d.ntile_s) return;
+    const int tid = threadIdx.x, grp = tid >> 4, l = tid & 15;
+    const size_t nB = (size_t)d.n;
+    const double2* __restrict__ x = k == 1 ? d.psi : d.tk_u + (size_t)((k - 1) & 1) * nB;
+    double2* __restrict__ y = d.tk_u + (size_t)(k & 1) * nB;
+    const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
+    const bool last = k == d.tk_p;
+    const double2 ck = last ? make_double2(0.0, 0.0) : d.tk_coef[k];
+    const int nact = d.nact[0];
+    double acc = 0.0;
+    const int r0 = rtile * d.rt_s, r1 = min(d.n, r0 + d.rt_s);
+    for (int i = r0 + grp; i < r1; i += 16) {
+        const int p0 = d.rowptr[i], p1 = d.rowptr[i + 1];
+        double sr = 0.0, si = 0.0;
+        for (int q = p0 + l; q < p1; q += 16) {
+            const double a = d.val64[q];
+            const double2 xv = x[d.col[q]];
+            sr += a * xv.x;
+            si += a * xv.y;
+        }
+        sr = group_sum(sr, 16);
+        si = group_sum(si, 16);
+        if (l == 0) {
+            const double2 xi = x[i];
+            const double di = (double)diag[i];
+            double wr = di * xi.x + d.g * sr;
+            double wi = di * xi.y + d.g * si;
+            if (i >= nact) { wr = 0.0; wi = 0.0; }
+            y[i] = make_double2(wr, wi);
+            if (!last) {
+                double2 tv = k == 1 ? xi : d.tk_t[i];
+                tv.x += ck.x * wr - ck.y * wi;
+                tv.y += ck.x * wi + ck.y * wr;
+                d.tk_t[i] = tv;
+            } else {
+                acc += wr * wr + wi * wi;
+            }
+        }
+    }
+    if (last) {
+        double out[1] = {acc};
+        store_partials<true, 1>(out, 1, d.part, d.ntile_s, rtile, 1, 0);
+    }
+}
+
+hipError_t launch_tk_power(const Dev& d, int k, hipStream_t s)
+{
+    if (d.plain) {
+        // the single-trajectory path is FP64 only
+        hipLaunchKernelGGL((k_tk_power_plain<double>), dim3(pad8(d.ntile_s)), dim3(kThreads), 0, s, d, k);
+    } else {
+        return launch_tk_power_b(d, k, s);
+    }
+    return hipGetLastError();
+}
+
+// |u_p| per trajectory from the partials of tk_power(p), one per SpMM tile
+template <bool PLAIN>
+__global__ __launch_bounds__(kThreads) void k_tk_norm(Dev d)
+{
+    const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const double s = block_tile_sum<PLAIN>(d.part, 0, d.ntile_s, d.Bp, b);
+    if (threadIdx.x < (PLAIN ? 1u : 64u)) d.tk_nrm[b] = __builtin_sqrt(s);
+}
+
+// basis slot 0 = u_p / |u_p| (calcAndStoreNormAndScale of the auxiliary
+// decomposition, taylorKrylovClass.cu:345)
+template <bool PLAIN, typename VT>
+__global__ __launch_bounds__(kThreads) void k_tk_scale(Dev d)
+{
+    using V2 = typename Vec2<VT>::T;
+    Tile t = elem_tile<PLAIN>(d);
+    if (!t.ok) return;
+    const int b = t.b;
+    const double nu = d.tk_nrm[b];
+    const double inv = nu > 0.0 ? 1.0 / nu : 0.0;
+    const double2* __restrict__ u = d.tk_u + (size_t)(d.tk_p & 1) * ((size_t)d.n * d.Bp);
+    V2* v0 = reinterpret_cast<V2*>(d.V);
+    const int rend = PLAIN ? t.r1 : min(t.r1, (int)uniform(d.nact[b & ~63]));
+    for (int i = t.rfirst; i < rend; i += t.rstep) {
+        const size_t e = (size_t)i * d.Bp + b;
+        const double2 a = u[e];
+        V2 v;
+        v.x = (VT)(a.x * inv);
+        v.y = (VT)(a.y * inv);
+        v0[e] = v;
+    }
+}
+
+hipError_t launch_tk_start(const Dev& d, hipStream_t s)
+{
+    dim3 grid(elem_grid(d)), block(kThreads);
+    if (d.plain) {
+        hipLaunchKernelGGL((k_tk_norm<true>), dim3(1), block, 0, s, d);
+        hipLaunchKernelGGL((k_tk_scale<true, double>), grid, block, 0, s, d);
+    } else {
+        hipLaunchKernelGGL((k_tk_norm<false>), dim3(d.sn / 64), block, 0, s, d);
+        if (d.mixed) hipLaunchKernelGGL((k_tk_scale<false, float>), grid, block, 0, s, d);
+        else hipLaunchKernelGGL((k_tk_scale<false, double>), grid, block, 0, s, d);
+    }
+    return hipGetLastError();
+}
+
+// psi = t + sum_{k<m} (y_k s_k) v_k, partial |psi|^2 (taylorKrylovClass.cu:372-418)
+template <bool PLAIN, typename VT, int MAXV>
+__global__ __launch_bounds__(kThreads) void k_tk_combine(Dev d)
+{
+    using V2 = typename Vec2<VT>::T;
+    __shared__ double2 s_y[MAXV][64];
+    Tile t = elem_tile<PLAIN>(d);
+    if (!t.ok) return;
+    const int b = t.b, bl = PLAIN ? 0 : (threadIdx.x & 63);
+    const int Bp = d.Bp, m = d.m;
+    if (PLAIN) {
+        if (threadIdx.x < (unsigned)m) s_y[threadIdx.x][0] = d.ycoef[threadIdx.x];
+    } else if (threadIdx.x < 64) {
+        for (int k = 0; k < m; ++k) s_y[k][bl] = d.ycoef[(size_t)k * Bp + b];
+    }
+    __syncthreads();
+    double2 y[MAXV];
+#pragma unroll
+    for (int k = 0; k < MAXV; ++k) y[k] = (k < m) ? s_y[k][bl] : make_double2(0.0, 0.0);
+    const size_t nB = (size_t)d.n * Bp;
+    const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
+    const int rend = PLAIN ? t.r1 : min(t.r1, (int)uniform(d.nact[b & ~63]));
+    double acc = 0.0;
+    for (int i = t.rfirst; i < rend; i += t.rstep) {
+        const size_t e = (size_t)i * Bp + b;
+        const double2 tv = d.tk_t[e];
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int k = 0; k < MAXV; ++k) {
+            if (k < m) {
+                const V2 v = V[(size_t)k * nB + e];
+                const double xr = (double)v.x, xi = (double)v.y;
+                ar += y[k].x * xr - y[k].y * xi;
+                ai += y[k].x * xi + y[k].y * xr;
+            }
+        }
+        ar = tv.x + ar;
+        ai = tv.y + ai;
+        d.psi[e] = make_double2(ar, ai);
+        acc += ar * ar + ai * ai;
+    }
+    double out[1] = {acc};
+    store_partials<PLAIN, 1>(out, 1, d.part, d.ntile_e, t.rtile, Bp, b);
+}
+
+template <bool PLAIN, typename VT>
+static void launch_tk_combine_t(const Dev& d, hipStream_t s)
+{
+    dim3 grid(elem_grid(d)), block(kThreads);
+    if (d.m <= 8) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 8>), grid, block, 0, s, d);
+    else if (d.m <= 16) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 16>), grid, block, 0, s, d);
+    else if (d.m <= 24) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 24>), grid, block, 0, s, d);
+    else hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 32>), grid, block, 0, s, d);
+}
+
+hipError_t launch_tk_combine(const Dev& d, hipStream_t s)
+{
+    if (!d.plain) return launch_combine_b(d, s);      // the batched combine with d.tk set
+    launch_tk_combine_t<true, double>(d, s);
     return hipGetLastError();
 }
 

@@ -19,6 +19,8 @@
  *   spectraClass::prepForCD / prepInitState  -> dq_cd_vectors
  *   systemClass::calcInvPatricRatio          -> dq_system_set_participation,
  *                                               dq_get_participation
+ *   taylorKrylovClass (PROP_MODE_TAYLOR_LANCZOS)
+ *                                            -> dq_system_set_taylor_krylov
  *   spectraClass::singleSpectraCalc          -> dq_spectrum
  * Every call returns a dq_status; the message of the last failure on the
  * calling thread is returned by dq_last_error().  Nothing prints, nothing
@@ -116,6 +118,21 @@ dq_status dq_system_set_zz_params(dq_system *sys, int32_t nosc, const double *om
  * / pi), kB = 0.695035 cm^-1/K (systemClass.cu:1118-1145). */
 dq_status dq_zz_oscillators(int32_t nosc, const double *omega_cm, const double *J, double domega,
                             double temperature_K, double *coeff);
+/* Taylor-Krylov propagation (propmode PROP_MODE_TAYLOR_LANCZOS,
+ * taylorKrylovClass.cu:133-434) instead of the Lanczos propagator: per Krylov
+ * cycle of length tau,
+ *   psi' = sum_{k<p} c^k A^k psi / k! + |A^p psi| Q exp(c Tbar_p)[0:m_aux, m_aux+p-1],
+ * c = -i tau / hbar, A the trajectory's Hamiltonian (noise included), Q and
+ * Tbar_p (the tridiagonal augmented by p, gpuLanczosHelper.cu:14-60) from
+ * m_aux Lanczos steps of the constant auxiliary matrix (rowptr, col, val
+ * off-diagonal CSR as in dq_system_set_hamiltonian_csr, diagonal diag[n]; the
+ * reference's tkmatfile) on A^p psi / |A^p psi|.  The error estimate per
+ * cycle is the auxiliary Expokit estimate + |1 - |psi'|| (clamped,
+ * taylorKrylovClass.cu:426-440).  Needs p >= 2, 2 <= m_aux <= m of the
+ * config, m_aux + p <= 34 (the reference's defaults: p = 10, m_aux = 24).
+ * p = 0 switches back to the Lanczos propagator.  Not with dq_adjoint. */
+dq_status dq_system_set_taylor_krylov(dq_system *sys, int32_t p, int32_t m_aux, const int32_t *rowptr,
+                                      const int32_t *col, const double *val, const double *diag);
 /* Philox key word 1 of each trajectory (NULL: 0..B-1) and its active site
  * count (NULL: all N).  A trajectory with n_active < N propagates under the
  * leading n_active x n_active block of H.  Clears observables and the state
@@ -338,7 +355,7 @@ typedef struct dq_infile {
     int32_t m;                  /* m */
     int32_t ka_substeps;        /* ka_substeps */
     int32_t noise_model;        /* simtype as a dq_noise value; CONST_HAM gives DQ_NOISE_NONE */
-    int32_t prop_mode;          /* propmode: 0 LANCZOS, 1 TAYLOR_KRYLOV */
+    int32_t prop_mode;          /* propmode: 0 PROP_MODE_LANCZOS, 1 PROP_MODE_TAYLOR_LANCZOS */
     int32_t static_dis;         /* staticdis */
     int32_t absorb_spec, cd_spec, lineshape_dress;
     int32_t state_looping, steps_per_loop;
@@ -348,6 +365,8 @@ typedef struct dq_infile {
     char ham_file[512], dipole_file[512], init_state_file[512], ka_params_file[512], hsr_file[512],
         zz_specd_file[512], out_folder[512];
     char pos_file[512];         /* posfile (circular dichroism) */
+    char tk_mat_file[512];      /* tkmatfile: Taylor-Krylov auxiliary matrix */
+    int32_t tk_p, tk_m_aux;     /* tk_p (default 10), tk_m_aux (default 24) */
 } dq_infile;
 dq_status dq_parse_infile(const char *path, dq_infile *out);
 

@@ -100,6 +100,13 @@ int orc_pade_expm(int n, const double *A, double *E, int *s_out);
 
 /* Expokit error estimate (padeClass.cu:749-782, systemClass.cu:2055-2085). */
 double orc_expokit_err(int m, double beta_m, double nrm_hq, const double *E);
+/* Augmented tridiagonal of size m + aug (gpuLanczosHelper.cu:14-60): T in the
+ * leading m x m block, 1 at (0, m) and at (i, i + 1) for m <= i < m + aug - 1;
+ * aug = 2 is orc_build_tbar.  Column-major. */
+void orc_build_tbar_aug(int m, int aug, const double *alpha, const double *beta, double *tb);
+/* Taylor coefficients c^k / k!, k = 0..p-1, of c = -i tau / hbar
+ * (taylorKrylovClass.cu:200-214), interleaved complex */
+void orc_tk_coeffs(int p, double tau, double hbar, double *coef);
 
 /* ---- trajectory ------------------------------------------------------ */
 typedef struct {
@@ -114,6 +121,12 @@ typedef struct {
     double g;              /* coupling scale */
     double sigma_static;   /* cm^-1 */
     uint64_t seed;
+    /* Taylor-Krylov propagation (taylorKrylovClass.cu:133-434) when tk_p > 0:
+     * order p of the Taylor part, m of the auxiliary Krylov part, and the
+     * auxiliary matrix (off-diagonal CSR + diagonal), read from tkmatfile */
+    int tk_p, tk_m;
+    const int32_t *tk_rowptr, *tk_col;
+    const double *tk_val, *tk_diag;
 } orc_params;
 
 /* site-energy noise models (systemClass.cu:1881-2013): Kubo-Anderson /
@@ -156,6 +169,12 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                   const int32_t *n_active, int64_t nsteps, int obs_every,
                   const int32_t *group, int ngroups, const double *bra,
                   double *pops, double *corr, double *partic, double *err_sum, int nthreads);
+
+/* One Taylor-Krylov cycle on psi (taylorKrylovClass.cu:217-434), A = the
+ * Hamiltonian (val, g, diag), the auxiliary matrix from p.  Returns the
+ * Expokit estimate of the auxiliary decomposition (unclamped) or < 0. */
+double orc_tk_cycle(int n, const int32_t *rowptr, const int32_t *col, const double *val, double g,
+                    const double *diag, const orc_params *p, double tau, double *psi);
 
 /* Static site energies e0 and initial OU offsets of trajectory traj
  * (systemClass.cu:759-804, 1028-1036). */

@@ -166,6 +166,21 @@ int orc_lanczos(int n, const int32_t *rowptr, const int32_t *col,
     return broke;
 }
 
+void orc_build_tbar_aug(int m, int aug, const double *alpha, const double *beta, double *tb)
+{
+    const int n = m + aug;
+    memset(tb, 0, sizeof(double) * n * n);
+    for (int i = 0; i < m; ++i) {
+        tb[i + i * n] = alpha[i + 1];
+        if (i + 1 < m) {
+            tb[i + (i + 1) * n] = beta[i + 1];
+            tb[(i + 1) + i * n] = beta[i + 1];
+        }
+    }
+    tb[0 + m * n] = 1.0;
+    for (int i = m; i + 1 < n; ++i) tb[i + (i + 1) * n] = 1.0;
+}
+
 void orc_build_tbar(int m, const double *alpha, const double *beta, double *tb)
 {
     const int n = m + 2;
@@ -305,9 +320,8 @@ int orc_pade_expm(int n, const double *A_in, double *E, int *s_out)
     return 0;
 }
 
-double orc_expokit_err(int m, double beta_m, double nrm_hq, const double *E)
+static double expokit_err_ld(int m, int n, double beta_m, double nrm_hq, const double *E)
 {
-    const int n = m + 2;
     double hm = fabs(beta_m);
     const double *e1 = E + 2 * ((m - 1) + m * n);
     const double *e2 = E + 2 * ((m - 1) + (m + 1) * n);
@@ -315,6 +329,11 @@ double orc_expokit_err(int m, double beta_m, double nrm_hq, const double *E)
     double err2 = hm * nrm_hq * sqrt(e2[0] * e2[0] + e2[1] * e2[1]);
     if (err1 > err2) return err2 / (1 - err2 / err1);
     return err1;
+}
+
+double orc_expokit_err(int m, double beta_m, double nrm_hq, const double *E)
+{
+    return expokit_err_ld(m, m + 2, beta_m, nrm_hq, E);
 }
 
 /* ------------------------------------------------------------------ */
@@ -449,6 +468,84 @@ static double krylov_cycle(int n, const int32_t *rowptr, const int32_t *col,
     return err;
 }
 
+void orc_tk_coeffs(int p, double tau, double hbar, double *coef)
+{
+    /* h_expCoeff = -i dt / hbar (propagate.cu:89-93); powComplex by repeated
+     * complex products from 1 (cppUtility.cpp:52-70), divided by k! */
+    const double cr = 0.0, ci = (-1. / hbar) * tau;
+    double zr = 1.0, zi = 0.0, fact = 1.0;
+    for (int k = 0; k < p; ++k) {
+        if (k > 0) {
+            const double r = zr * cr - zi * ci, i = zr * ci + zi * cr;
+            zr = r;
+            zi = i;
+            fact *= (double)k;
+        }
+        coef[2 * k] = zr / fact;
+        coef[2 * k + 1] = zi / fact;
+    }
+}
+
+double orc_tk_cycle(int n, const int32_t *rowptr, const int32_t *col, const double *val, double g,
+                    const double *diag, const orc_params *p, double tau, double *psi)
+{
+    const int P = p->tk_p, m = p->tk_m, nt = m + P;
+    const size_t cs = 2 * (size_t)n;
+    double *t = (double *)malloc(cs * sizeof(double) * 3);
+    double *u = t + cs, *y = u + cs;
+    double *Q = (double *)malloc(cs * (m + 2) * sizeof(double));
+    double *coef = (double *)malloc(sizeof(double) * 2 * P);
+    double *alpha = (double *)malloc(sizeof(double) * (2 * (m + 2) + nt * nt * 5));
+    double *beta = alpha + (m + 2), *tb = beta + (m + 2), *A = tb + nt * nt, *E = A + 2 * nt * nt;
+    orc_tk_coeffs(P, tau, p->hbar, coef);
+    /* Taylor part: t = sum_{k<p} c^k A^k psi / k!, u = A^p psi
+     * (taylorKrylovClass.cu:226-322) */
+    memcpy(t, psi, cs * sizeof(double));
+    memcpy(u, psi, cs * sizeof(double));
+    for (int k = 1; k <= P; ++k) {
+        orc_spmv(n, rowptr, col, val, g, diag, u, y);
+        if (k < P) {
+            const double ar = coef[2 * k], ai = coef[2 * k + 1];
+            for (int i = 0; i < n; ++i) {
+                t[2 * i] += ar * y[2 * i] - ai * y[2 * i + 1];
+                t[2 * i + 1] += ar * y[2 * i + 1] + ai * y[2 * i];
+            }
+        }
+        memcpy(u, y, cs * sizeof(double));
+    }
+    /* Krylov part on the auxiliary matrix from A^p psi / |A^p psi|
+     * (taylorKrylovClass.cu:345-418, calcAndStoreNormAndScale) */
+    const double nu = cnorm(n, u);
+    const double inv = nu > 0.0 ? 1.0 / nu : 0.0;
+    for (size_t i = 0; i < cs; ++i) u[i] *= inv;
+    orc_lanczos_opts lo = {m, p->reorth, p->breakdown_tol};
+    double nrm_hq;
+    orc_lanczos(n, p->tk_rowptr, p->tk_col, p->tk_val, 1.0, p->tk_diag, u, &lo, Q, alpha, beta, &nrm_hq);
+    orc_build_tbar_aug(m, P, alpha, beta, tb);
+    const double cim = (-1. / p->hbar) * tau;
+    for (int i = 0; i < nt * nt; ++i) { A[2 * i] = 0.0; A[2 * i + 1] = cim * tb[i]; }
+    int s;
+    double err = -1.0;
+    if (orc_pade_expm(nt, A, E, &s) == 0) {
+        /* psi = t + |A^p psi| Q_m E[0:m, m+p-1] (taylorKrylovClass.cu:372-418) */
+        const double *col_last = E + 2 * (size_t)nt * (nt - 1);
+        for (int i = 0; i < n; ++i) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 1; k <= m; ++k) {
+                const double *q = Q + cs * k + 2 * i;
+                const double yr = col_last[2 * (k - 1)], yi = col_last[2 * (k - 1) + 1];
+                sr += yr * q[0] - yi * q[1];
+                si += yr * q[1] + yi * q[0];
+            }
+            psi[2 * i] = t[2 * i] + nu * sr;
+            psi[2 * i + 1] = t[2 * i + 1] + nu * si;
+        }
+        err = expokit_err_ld(m, nt, beta[m], nrm_hq, E);
+    }
+    free(t); free(Q); free(coef); free(alpha);
+    return err;
+}
+
 static void record(int n, const double *psi, const int32_t *group, int ngroups, double *out)
 {
     for (int g = 0; g < ngroups; ++g) out[g] = 0.0;
@@ -498,11 +595,23 @@ int orc_run(const orc_params *p, const orc_site_params *sp,
         orc_noise_step(p, sp, traj, step, de);
         for (int i = 0; i < n; ++i) diag[i] = de[i] + e0[i];
         for (int k = 0; k < p->krylov_substeps; ++k) {
-            double err = krylov_cycle(n, rowptr, col, val, p->g, diag, &lo, tau,
-                                      p->hbar, psi, Q, work);
-            if (err < 0.0) { rc = -1; goto done; }
-            esum += err;
+            double err;
+            if (p->tk_p > 0) {
+                err = orc_tk_cycle(n, rowptr, col, val, p->g, diag, p, tau, psi);
+                if (err < 0.0) { rc = -1; goto done; }
+            } else {
+                err = krylov_cycle(n, rowptr, col, val, p->g, diag, &lo, tau,
+                                   p->hbar, psi, Q, work);
+                if (err < 0.0) { rc = -1; goto done; }
+            }
             double nr = cnorm(n, psi);
+            if (p->tk_p > 0) {
+                /* getTaylorKrylovError adds |1 - |psi||, then the clamp
+                 * (taylorKrylovClass.cu:426-440, systemClass.cu:2071-2077) */
+                err = err + fabs(1. - nr);
+                if (err < 1.110223e-16) err = 1.110223e-16;
+            }
+            esum += err;
             double inv = 1.0 / nr;
             for (size_t i = 0; i < cs; ++i) psi[i] *= inv;
         }
@@ -550,6 +659,27 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
         for (int i = 0; i < nb_act; ++i)
             for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q)
                 if (col[q] < nb_act) { cl[cnt] = col[q]; vl[cnt] = val[q]; ++cnt; }
+        /* the auxiliary Taylor-Krylov matrix, cut to the same leading block */
+        int32_t *trp = NULL, *tcl = NULL;
+        double *tvl = NULL;
+        if (p->tk_p > 0) {
+            trp = (int32_t *)malloc(sizeof(int32_t) * (nb_act + 1));
+            int64_t tc = 0;
+            trp[0] = 0;
+            for (int i = 0; i < nb_act; ++i) {
+                for (int32_t q = p->tk_rowptr[i]; q < p->tk_rowptr[i + 1]; ++q) if (p->tk_col[q] < nb_act) ++tc;
+                trp[i + 1] = (int32_t)tc;
+            }
+            tcl = (int32_t *)malloc(sizeof(int32_t) * (tc + 1));
+            tvl = (double *)malloc(sizeof(double) * (tc + 1));
+            tc = 0;
+            for (int i = 0; i < nb_act; ++i)
+                for (int32_t q = p->tk_rowptr[i]; q < p->tk_rowptr[i + 1]; ++q)
+                    if (p->tk_col[q] < nb_act) { tcl[tc] = p->tk_col[q]; tvl[tc] = p->tk_val[q]; ++tc; }
+            pb.tk_rowptr = trp;
+            pb.tk_col = tcl;
+            pb.tk_val = tvl;
+        }
         int r = orc_run(&pb, sp, rp, cl, vl, psi0 + (size_t)b * 2 * n, traj_ids[b], nsteps,
                         obs_every, group, ngroups, bra,
                         pops ? pops + (size_t)b * nobs * ngroups : NULL,
@@ -558,6 +688,7 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                         NULL, err_sum ? err_sum + b : NULL);
         if (r != 0) fail = 1;
         free(rp); free(cl); free(vl);
+        free(trp); free(tcl); free(tvl);
     }
     return fail ? -1 : 0;
 }

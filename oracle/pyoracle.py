@@ -28,6 +28,8 @@ class Params(C.Structure):
         ("breakdown_tol", C.c_double), ("dt_fs", C.c_double),
         ("hbar", C.c_double), ("g", C.c_double),
         ("sigma_static", C.c_double), ("seed", C.c_uint64),
+        ("tk_p", C.c_int), ("tk_m", C.c_int), ("tk_rowptr", C.c_void_p), ("tk_col", C.c_void_p),
+        ("tk_val", C.c_void_p), ("tk_diag", C.c_void_p),
     ]
 
 
@@ -92,6 +94,11 @@ def lib():
                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_participation.restype = C.c_double
         L.orc_participation.argtypes = [C.c_int, _dp]
+        L.orc_tk_coeffs.argtypes = [C.c_int, C.c_double, C.c_double, _dp]
+        L.orc_build_tbar_aug.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_tk_cycle.restype = C.c_double
+        L.orc_tk_cycle.argtypes = [C.c_int, _ip, _ip, _dp, C.c_double, _dp, C.POINTER(Params),
+                                   C.c_double, _dp]
         L.orc_run_batch.restype = C.c_int
         L.orc_run_batch.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), _ip, _ip, _dp, _dp,
                                     _up, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
@@ -254,6 +261,40 @@ def make_params(n, m, dt, K=1, S=1, reorth=REORTH_CGS, tol=1e-10, hbar=None, g=1
                 sigma_static=0.0, seed=1):
     return Params(n, m, K, S, reorth, tol, dt, hbar_ref() if hbar is None else hbar, g,
                   sigma_static, seed)
+
+
+def set_taylor_krylov(p, tk_p, tk_m, rowptr, col, val, diag):
+    """switch p to Taylor-Krylov propagation with the auxiliary matrix
+    (rowptr, col, val) + diag; returns the arrays, which must stay alive as
+    long as p is used"""
+    keep = (_c(np.asarray(rowptr, np.int32)), _c(np.asarray(col, np.int32)),
+            _c(np.asarray(val, np.float64)), _c(np.asarray(diag, np.float64)))
+    p.tk_p, p.tk_m = tk_p, tk_m
+    p.tk_rowptr, p.tk_col, p.tk_val, p.tk_diag = (a.ctypes.data for a in keep)
+    return keep
+
+
+def tk_coeffs(p, tau, hbar=None):
+    out = np.zeros(2 * p)
+    lib().orc_tk_coeffs(p, tau, hbar_ref() if hbar is None else hbar, out)
+    return i2c(out)
+
+
+def build_tbar_aug(m, aug, alpha, beta):
+    """(m + aug)^2 augmented tridiagonal from alpha[1..m], beta[1..m-1]
+    (index 0 unused, as orc_lanczos returns them), row-major numpy"""
+    n = m + aug
+    tb = np.zeros(n * n)
+    lib().orc_build_tbar_aug(m, aug, _c(np.asarray(alpha, np.float64)), _c(np.asarray(beta, np.float64)), tb)
+    return tb.reshape(n, n).T
+
+
+def tk_cycle(p, rowptr, col, val, diag, psi, tau):
+    """one Taylor-Krylov cycle: (psi', unclamped Krylov error estimate)"""
+    a = c2i(np.asarray(psi, np.complex128))
+    err = lib().orc_tk_cycle(p.n, _c(rowptr), _c(col), _c(val), p.g, _c(np.asarray(diag, np.float64)),
+                             C.byref(p), tau, a)
+    return i2c(a), err
 
 
 class Sites:

@@ -24,6 +24,18 @@ def test_parse_reference_infile(engine):
     assert r["zz_specd_file"] == "data-example/specd_tresp.dat"
     assert r["init_state_file"] == "data-example/sample_multisite_init.dat"
     assert r["out_folder"] == "outDataFolder" and r["prop_mode"] == 0
+    assert r["tk_p"] == 10 and r["tk_m_aux"] == 24 and r["tk_mat_file"] == ""    # systemClass.cu:70-71
+
+
+def test_infile_taylor_krylov_keys(engine, tmp_path):
+    f = tmp_path / "tk.in"
+    f.write_text("propmode = PROP_MODE_TAYLOR_LANCZOS\ntkmatfile = data/hamB.dat\ntk_p = 6\ntk_m_aux = 16\n")
+    r = engine.parse_infile(str(f))
+    assert r["prop_mode"] == 1 and r["tk_mat_file"] == "data/hamB.dat"
+    assert r["tk_p"] == 6 and r["tk_m_aux"] == 16
+    f.write_text("propmode = PROP_MODE_TAYLOR_KRYLOV\n")     # not a mode the reference knows
+    with pytest.raises(engine.DQError):
+        engine.parse_infile(str(f))
 
 
 def test_infile_forms_and_errors(engine, tmp_path):
