@@ -7,9 +7,10 @@ namespace dq {
 struct AdjDev {
     int m, K, last_c;
     size_t nB;                 // n * Bp
+    size_t vs;                 // stride of the basis slots in the tape (the engine's Dev::vs)
     int Bp;
     // tape of one checkpoint segment, cycle c = (step - s0) * K + k
-    double2* tapeV;            // [ncyc][m+1][nB] basis (slot 0 = cycle input psi)
+    double2* tapeV;            // [ncyc][m+1][vs] basis (slot 0 = cycle input psi)
     double* sc;                // [ncyc][m+1][Bp]
     double* alpha;             // [ncyc][m][Bp]
     double* beta;              // [ncyc][m][Bp]
@@ -33,7 +34,7 @@ struct AdjDev {
     size_t pade_stride;
     int max_s;
 
-    __host__ __device__ const double2* basis(int c) const { return tapeV + (size_t)c * (m + 1) * nB; }
+    __host__ __device__ const double2* basis(int c) const { return tapeV + (size_t)c * (m + 1) * vs; }
     __host__ __device__ const double2* psi_out(int c) const { return c < last_c ? basis(c + 1) : final_psi; }
     __host__ __device__ double2* qbar_slot(int k) const { return qb[k % 3]; }
     __host__ __device__ const double* diag_step(int c) const { return tdiag + (size_t)(c / K) * nB; }

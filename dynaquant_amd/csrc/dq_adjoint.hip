@@ -114,7 +114,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_renorm_apply(Dev d, AdjDev a, 
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
             if (k < m) {
-                const double2 q = V[(size_t)k * nB + e];
+                const double2 q = V[(size_t)k * a.vs + e];
                 acc[2 * u] += q.x * v.x + q.y * v.y;
                 acc[2 * u + 1] += q.x * v.y - q.y * v.x;
             }
@@ -410,14 +410,14 @@ __global__ __launch_bounds__(kThreads) void k_adj_wbar(Dev d, AdjDev a, int c, i
         double2 w = make_double2(0.0, 0.0);
         if (coef_qb != 0.0) {
             const double2 qb = qb1[e];
-            const double2 v1 = V[(size_t)(j + 1) * nB + e];
+            const double2 v1 = V[(size_t)(j + 1) * a.vs + e];
             w = make_double2(coef_qb * qb.x + coef_q * v1.x, coef_qb * qb.y + coef_q * v1.y);
         }
         a.wb[e] = w;
-        const double2 vj = V[(size_t)j * nB + e];
+        const double2 vj = V[(size_t)j * a.vs + e];
         acc[0] += sj * (vj.x * w.x + vj.y * w.y);
         if (j >= 1) {
-            const double2 vm = V[(size_t)(j - 1) * nB + e];
+            const double2 vm = V[(size_t)(j - 1) * a.vs + e];
             acc[1] += sjm * (vm.x * w.x + vm.y * w.y);
         }
     }
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm(Dev d, AdjDev a, int c, i
     const double abart = a.abar[(size_t)j * Bp + b] - a.r2[b];
     const double2 yj = a.y[(size_t)j * Bp + b];
     const double2 yjm = j >= 1 ? a.y[(size_t)(j - 1) * Bp + b] : make_double2(0.0, 0.0);
-    const double2* vj = V + (size_t)j * nB;
+    const double2* vj = V + (size_t)j * a.vs;
     double2* qbj = a.qbar_slot(j);
     double2* qbm = j >= 1 ? a.qbar_slot(j - 1) : nullptr;
     const double cq = abart * sj;                 // wbar = wbar' + cq v_j
@@ -489,10 +489,10 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm(Dev d, AdjDev a, int c, i
         const double di = diag[e];
         const double2 Hw = make_double2(di * wbar.x + g * sr, di * wbar.y + g * si);
         // H q_j by the three-term relation
-        const double2 x1 = V[(size_t)(j + 1) * nB + e];
+        const double2 x1 = V[(size_t)(j + 1) * a.vs + e];
         double2 hq = make_double2(bj * sj1 * x1.x + aj * q.x, bj * sj1 * x1.y + aj * q.y);
         if (j >= 1) {
-            const double2 xm = V[(size_t)(j - 1) * nB + e];
+            const double2 xm = V[(size_t)(j - 1) * a.vs + e];
             hq.x += bjm * sjm * xm.x;
             hq.y += bjm * sjm * xm.y;
         }
@@ -618,12 +618,12 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
             if (i >= d.n || i >= nact) continue;
             const size_t e = (size_t)i * Bp + b;
             const double di = diag[e];
-            const double2 xj = V[(size_t)j * nB + e];
+            const double2 xj = V[(size_t)j * a.vs + e];
             const double2 q = make_double2(sj * xj.x, sj * xj.y);
-            const double2 x1 = V[(size_t)(j + 1) * nB + e];
+            const double2 x1 = V[(size_t)(j + 1) * a.vs + e];
             const double2 q1 = make_double2(sj1 * x1.x, sj1 * x1.y);
             double2 qm = make_double2(0.0, 0.0);
-            if (j >= 1) { const double2 t = V[(size_t)(j - 1) * nB + e]; qm = make_double2(sjm * t.x, sjm * t.y); }
+            if (j >= 1) { const double2 t = V[(size_t)(j - 1) * a.vs + e]; qm = make_double2(sjm * t.x, sjm * t.y); }
             // H q_j by the three-term relation
             const double2 hq = make_double2(bj * q1.x + aj * q.x + bjm * qm.x, bj * q1.y + aj * q.y + bjm * qm.y);
             double2 wp = make_double2(0.0, 0.0), Jw = make_double2(0.0, 0.0);
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(kThreads) void k_adj_spmm_rb(Dev d, AdjDev a, int c
                 const double2 qbn = qb1[e];
                 wp = make_double2(cqb * qbn.x + btot * q1.x, cqb * qbn.y + btot * q1.y);
                 // H q_{j+1} by the three-term relation (slot j+2 <= m holds q_{j+2})
-                const double2 x2 = V[(size_t)(j + 2) * nB + e];
+                const double2 x2 = V[(size_t)(j + 2) * a.vs + e];
                 const double2 hq1 = make_double2(bj1 * sj2 * x2.x + aj1 * q1.x + bj * q.x,
                                                  bj1 * sj2 * x2.y + aj1 * q1.y + bj * q.y);
                 Jw = make_double2(cqb * sr[r] + btot * (hq1.x - di * q1.x) * ginv,

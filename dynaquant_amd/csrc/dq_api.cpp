@@ -313,8 +313,9 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     void* diag = nullptr;
 #define ALLOC(p, cnt) do { if ((st = s->alloc(p, cnt)) != DQ_OK) { delete s; return st; } } while (0)
     ALLOC(&psi, nB);
-    if (d.mixed) { float2* v; ALLOC(&v, nB * (s->m + 1)); V = v; float* dg; ALLOC(&dg, nB); diag = dg; }
-    else { double2* v; ALLOC(&v, nB * (s->m + 1)); V = v; double* dg; ALLOC(&dg, nB); diag = dg; }
+    d.vs = (nB + 63) / 64 * 64 + 32;
+    if (d.mixed) { float2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; float* dg; ALLOC(&dg, nB); diag = dg; }
+    else { double2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; double* dg; ALLOC(&dg, nB); diag = dg; }
     d.psi = psi; d.V = V; d.diag = diag;
     ALLOC(&d.de, nB);
     ALLOC(&d.e0, nB);
@@ -677,7 +678,7 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
     // write rows that are inactive for a whole slice, so clear them all here
     const size_t nB = (size_t)d.n * d.Bp;
     const size_t vb = d.mixed ? sizeof(float2) : sizeof(double2);
-    HIP_TRY(hipMemsetAsync(d.V, 0, vb * nB * (sys->m + 1), sys->stream));
+    HIP_TRY(hipMemsetAsync(d.V, 0, vb * d.vs * (sys->m + 1), sys->stream));
     HIP_TRY(hipMemsetAsync(d.psi, 0, sizeof(double2) * nB, sys->stream));
     if (d.tk_t) {
         HIP_TRY(hipMemsetAsync(d.tk_t, 0, sizeof(double2) * nB, sys->stream));
@@ -1166,7 +1167,7 @@ dq_status dq_spmv(dq_system* sys, int32_t traj, const double* x, double* y)
         HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(float2) * nB, hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
         HIP_TRY(hipStreamSynchronize(sys->stream));
-        HIP_TRY(hipMemcpy(h.data(), (float2*)d.V + nB, sizeof(float2) * nB, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), (float2*)d.V + d.vs, sizeof(float2) * nB, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) { y[2 * i] = h[(size_t)i * Bp + traj].x; y[2 * i + 1] = h[(size_t)i * Bp + traj].y; }
     } else {
         std::vector<double2> h(nB, make_double2(0.0, 0.0));
@@ -1174,7 +1175,7 @@ dq_status dq_spmv(dq_system* sys, int32_t traj, const double* x, double* y)
         HIP_TRY(hipMemcpyAsync(d.V, h.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(dq::launch_spmm(d, 0, 0, sys->stream));
         HIP_TRY(hipStreamSynchronize(sys->stream));
-        HIP_TRY(hipMemcpy(h.data(), (double2*)d.V + nB, sizeof(double2) * nB, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), (double2*)d.V + d.vs, sizeof(double2) * nB, hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) { y[2 * i] = h[(size_t)i * Bp + traj].x; y[2 * i + 1] = h[(size_t)i * Bp + traj].y; }
     }
     return DQ_OK;
@@ -1300,7 +1301,7 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     };
     auto cleanup = [&]() { hipStreamSynchronize(st); for (void* p : tmp) hipFree(p); };
     dq::AdjDev a{};
-    a.m = m; a.K = K; a.nB = nB; a.Bp = Bp; a.max_s = max_s;
+    a.m = m; a.K = K; a.nB = nB; a.vs = d.vs; a.Bp = Bp; a.max_s = max_s;
     double2 *ck_psi, *ad_pbar;
     double *ck_de, *ck_nrm;
     dq_status rc = DQ_OK;
@@ -1308,7 +1309,7 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     AGET(&ck_psi, sizeof(double2) * nB * nck);
     AGET(&ck_de, sizeof(double) * nB * nck);
     AGET(&ck_nrm, sizeof(double) * Bp * nck);
-    AGET(&a.tapeV, sizeof(double2) * nB * (m + 1) * ncyc);
+    AGET(&a.tapeV, sizeof(double2) * d.vs * (m + 1) * ncyc);
     AGET(&a.sc, sizeof(double) * Bp * (m + 1) * ncyc);
     AGET(&a.alpha, sizeof(double) * Bp * m * ncyc);
     AGET(&a.beta, sizeof(double) * Bp * m * ncyc);
@@ -1334,7 +1335,7 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     // into the tape
     auto tape_dev = [&](int c) {
         Dev dc = d;
-        dc.V = a.tapeV + (size_t)c * (m + 1) * nB;
+        dc.V = a.tapeV + (size_t)c * (m + 1) * d.vs;
         dc.sc = a.sc + (size_t)c * (m + 1) * Bp;
         dc.alpha = a.alpha + (size_t)c * m * Bp;
         dc.beta = a.beta + (size_t)c * m * Bp;
@@ -1482,8 +1483,8 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "spmm_phases must be 1, 2 or 4 and divide the number of trajectory slices";
         return false;
     }
-    if (t.tile_rows != 64 && t.tile_rows != 128 && t.tile_rows != 256) {
-        *why = "tile_rows must be 64, 128 or 256";
+    if (t.tile_rows != 32 && t.tile_rows != 64 && t.tile_rows != 128 && t.tile_rows != 256) {
+        *why = "tile_rows must be 32, 64, 128 or 256";
         return false;
     }
     if (t.spmm_pipeline != 0 && t.spmm_pipeline != 1) {
@@ -1666,15 +1667,16 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     for (int R : {4, 2, 8}) {
         for (int tpl : {2, 1}) {
             for (int ph : {1, 2, 4}) {
-                for (int cand = 0; cand < 6; ++cand) {
+                for (int cand = 0; cand < 8; ++cand) {
                     if (e != hipSuccess || st != DQ_OK) break;
                     dq_tuning t;
                     std::memset(&t, 0, sizeof(t));
                     t.spmm_rows_per_wave = R;
                     t.spmm_traj_per_lane = tpl;
                     t.spmm_phases = ph;
-                    t.tile_rows = cand % 3 == 0 ? 128 : (cand % 3 == 1 ? 64 : 256);
-                    t.spmm_pipeline = cand / 3;
+                    static const int rows[4] = {128, 64, 256, 32};
+                    t.tile_rows = rows[cand % 4];
+                    t.spmm_pipeline = cand / 4;
                     std::string why;
                     if (!tuning_valid(sys, t, &why)) continue;
                     if ((st = apply_tuning(sys, t)) != DQ_OK) break;

@@ -133,9 +133,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
     const int Bp = d.Bp;
     const int b0 = ttile * TW + lane * TPL;              // first trajectory of this lane
     const size_t nB = (size_t)d.n * Bp;
-    const V2* __restrict__ xs = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB + (size_t)ttile * TW;
-    V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
-    const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * nB;
+    const V2* __restrict__ xs = reinterpret_cast<const V2*>(d.V) + (size_t)j * d.vs + (size_t)ttile * TW;
+    V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * d.vs;
+    const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * d.vs;
     const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
     const VT* __restrict__ rv;
     if constexpr (sizeof(VT) == 4) rv = d.rb_val32;
@@ -511,6 +511,23 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     const int b = ttile * 16 + t;
     const int Bp = d.Bp;
     const size_t nB = (size_t)d.n * Bp;
+    const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
+    V2* __restrict__ out = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * d.vs;
+    // rows at or beyond the first (largest) n_active of the 16 sorted slots
+    // are inactive for all of them: skipped, they stay 0
+    const int r0 = rtile * d.rt_e, r1 = min(min(d.n, r0 + d.rt_e), (int)uniform(d.nact[ttile * 16]));
+    // software pipeline: the vectors of row i+4 load while row i is combined;
+    // the first row's loads are issued before the coefficients are staged
+    V2 nk[KW], nw;
+    if (r0 + wave < r1) {
+        const size_t e = (size_t)(r0 + wave) * Bp + b;
+#pragma unroll
+        for (int u = 0; u < KW; ++u) {
+            const int k = kg + 4 * u;
+            if (k <= j) nk[u] = V[(size_t)k * d.vs + e];
+        }
+        if (kg == 0) nw = out[e];
+    }
     for (int q = threadIdx.x; q < 16 * (j + 1); q += kThreads) {
         const int k = q >> 4, tt = q & 15, bb = ttile * 16 + tt;
         const double2 h = d.hco[((size_t)j * kMaxM + k) * Bp + bb];
@@ -519,8 +536,6 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
         if (k == 0) s_w[tt] = d.sc[(size_t)j * Bp + bb];
     }
     __syncthreads();
-    const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
-    V2* __restrict__ out = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
     const double sw = s_w[t];
     double2 cf[KW];
 #pragma unroll
@@ -531,20 +546,6 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     double acc[2 * KW + 1];
 #pragma unroll
     for (int q = 0; q < 2 * KW + 1; ++q) acc[q] = 0.0;
-    // rows at or beyond the first (largest) n_active of the 16 sorted slots
-    // are inactive for all of them: skipped, they stay 0
-    const int r0 = rtile * d.rt_e, r1 = min(min(d.n, r0 + d.rt_e), (int)uniform(d.nact[ttile * 16]));
-    // software pipeline: the vectors of row i+4 load while row i is combined
-    V2 nk[KW], nw;
-    if (r0 + wave < r1) {
-        const size_t e = (size_t)(r0 + wave) * Bp + b;
-#pragma unroll
-        for (int u = 0; u < KW; ++u) {
-            const int k = kg + 4 * u;
-            if (k <= j) nk[u] = V[(size_t)k * nB + e];
-        }
-        if (kg == 0) nw = out[e];
-    }
     for (int i = r0 + wave; i < r1; i += 4) {
         const size_t e = (size_t)i * Bp + b;
         V2 vk[KW];
@@ -556,7 +557,7 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
 #pragma unroll
             for (int u = 0; u < KW; ++u) {
                 const int k = kg + 4 * u;
-                if (k <= j) nk[u] = V[(size_t)k * nB + en];
+                if (k <= j) nk[u] = V[(size_t)k * d.vs + en];
             }
             if (kg == 0) nw = out[en];
         }
@@ -681,7 +682,7 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k >= k0 && k < m) nk[u] = V[(size_t)k * nB + e];
+            if (k >= k0 && k < m) nk[u] = V[(size_t)k * d.vs + e];
         }
         if (kg == 0) np = TK ? d.tk_t[e] : d.psi[e];
     }
@@ -696,7 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
             for (int u = 0; u < KW; ++u) {
                 const int k = kg + 4 * u;
-                if (k >= k0 && k < m) nk[u] = V[(size_t)k * nB + en];
+                if (k >= k0 && k < m) nk[u] = V[(size_t)k * d.vs + en];
             }
             if (kg == 0) np = TK ? d.tk_t[en] : d.psi[en];
         }

@@ -15,7 +15,7 @@ constexpr int kMaxM = 32;          // Krylov depth limit (Pade matrix <= 34)
 constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
 constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
-constexpr int kRowsMin = 64;       // smallest elementwise tile (rows per workgroup)
+constexpr int kRowsMin = 32;       // smallest elementwise tile (rows per workgroup)
 
 struct Dev {
     int n, Bp, B, m;
@@ -61,7 +61,10 @@ struct Dev {
     const uint64_t* traj_id;        // [Bp]
     // state
     double2* psi;                   // [n][Bp]
-    void* V;                        // (m+1) x [n][Bp] of float2 / double2
+    void* V;                        // (m+1) slots of [n][Bp] float2 / double2, vs elements apart
+    size_t vs;                      // slot stride: n Bp rounded up to 64, + 32, so that the
+                                    // slots are an odd multiple of 256 B apart (a multiple of
+                                    // 16 KB puts every slot's row i on the same HBM channels)
     void* diag;                     // [n][Bp] float / double
     double* de;                     // [n][Bp] OU offsets
     double* e0;                     // [n][Bp] static site energies

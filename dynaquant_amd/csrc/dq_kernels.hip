@@ -366,8 +366,8 @@ __global__ __launch_bounds__(kThreads) void k_spmv_plain(Dev d, int j, int last)
     if (rtile >= d.ntile_s) return;
     const int tid = threadIdx.x, grp = tid >> 4, l = tid & 15;
     const size_t nB = (size_t)d.n;
-    const double2* __restrict__ x = reinterpret_cast<const double2*>(d.V) + (size_t)j * nB;
-    double2* __restrict__ y = reinterpret_cast<double2*>(d.V) + (size_t)(j + 1) * nB;
+    const double2* __restrict__ x = reinterpret_cast<const double2*>(d.V) + (size_t)j * d.vs;
+    double2* __restrict__ y = reinterpret_cast<double2*>(d.V) + (size_t)(j + 1) * d.vs;
     const double* __restrict__ diag = reinterpret_cast<const double*>(d.diag);
     const int nact = d.nact[0];
     double acc0 = 0.0, acc1 = 0.0;
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(kThreads) void k_update(Dev d, int j)
     const double sw = s_w[bl];
     const size_t nB = (size_t)d.n * Bp;
     const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
-    V2* __restrict__ out = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * nB;
+    V2* __restrict__ out = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * d.vs;
     double acc[2 * MAXV + 1];
 #pragma unroll
     for (int q = 0; q < 2 * MAXV + 1; ++q) acc[q] = 0.0;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(kThreads) void k_update(Dev d, int j)
         V2 vk[MAXV];
 #pragma unroll
         for (int k = 0; k < MAXV; ++k)
-            if (k <= j) vk[k] = V[(size_t)k * nB + e];
+            if (k <= j) vk[k] = V[(size_t)k * d.vs + e];
         const V2 w = out[e];
         double ar = sw * (double)w.x, ai = sw * (double)w.y;
 #pragma unroll
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(kThreads) void k_combine(Dev d)
 #pragma unroll
         for (int k = 1; k < MAXV; ++k) {
             if (k < m) {
-                const V2 v = V[(size_t)k * nB + e];
+                const V2 v = V[(size_t)k * d.vs + e];
                 const double xr = (double)v.x, xi = (double)v.y;
                 ar += y[k].x * xr - y[k].y * xi;
                 ai += y[k].x * xi + y[k].y * xr;
@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(kThreads) void k_tk_combine(Dev d)
 #pragma unroll
         for (int k = 0; k < MAXV; ++k) {
             if (k < m) {
-                const V2 v = V[(size_t)k * nB + e];
+                const V2 v = V[(size_t)k * d.vs + e];
                 const double xr = (double)v.x, xi = (double)v.y;
                 ar += y[k].x * xr - y[k].y * xi;
                 ai += y[k].x * xi + y[k].y * xr;

@@ -284,7 +284,7 @@ typedef struct dq_tuning {
     int32_t spmm_rows_per_wave;  /* 2, 4 or 8 rows whose column union one wavefront gathers */
     int32_t spmm_traj_per_lane;  /* 1 or 2 (2: FP32 only, padded batch a multiple of 128) */
     int32_t spmm_phases;         /* trajectory slices of one SpMM run as 1, 2 or 4 launches */
-    int32_t tile_rows;           /* 64, 128 or 256 rows per workgroup of the update, combine
+    int32_t tile_rows;           /* 32, 64, 128 or 256 rows per workgroup of the update, combine
                                     and noise kernels */
     int32_t spmm_pipeline;       /* 0 or 1: gathers of the next 8 union columns in flight while
                                     the current 8 are multiplied in */

@@ -61,7 +61,7 @@ def test_tile_rows_fp64_against_oracle(engine, oracle):
     grp = (np.arange(n) // 150).astype(np.int32)
     rb = oracle.run_batch(p, sites, sysd["rp"], sysd["col"], sysd["val"], psi0, np.arange(B), steps, 5,
                           grp, 4)
-    for rows in (64, 128, 256):
+    for rows in (32, 64, 128, 256):
         for rpw in (2, 8):
             pops, _, _ = _run(engine, sysd, engine.FP64, B,
                               dict(tile_rows=rows, spmm_rows_per_wave=rpw, spmm_traj_per_lane=1))
