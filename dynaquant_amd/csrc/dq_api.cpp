@@ -172,6 +172,9 @@ struct dq_system {
 
 extern "C" const char* dq_last_error(void) { return g_err.c_str(); }
 
+// R = 16 (the matrix-core SpMM, four union columns per step, four steps in
+// flight): every block's union is padded to a multiple of 16 with zero-valued
+// copies of its first column.
 static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_t>& rowptr_v,
                                  const std::vector<int32_t>& col_v, const std::vector<double>& val_v, RowBlocks* out)
 {
@@ -196,6 +199,10 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
                 const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
                 rbv64[(base + pos) * R + (i - r0)] = val[p];
             }
+        if (R == 16) {
+            while ((rbc.size() - base) % 16) rbc.push_back(u[0]);
+            rbv64.resize(rbc.size() * R, 0.0);
+        }
         rbp[k + 1] = (int32_t)rbc.size();
     }
     std::vector<float> rbv32(rbv64.size());
@@ -1466,19 +1473,34 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
 static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* why)
 {
     const Dev& d = sys->d;
-    if (t.spmm_rows_per_wave != 2 && t.spmm_rows_per_wave != 4 && t.spmm_rows_per_wave != 8) {
-        *why = "spmm_rows_per_wave must be 2, 4 or 8";
+    if (t.spmm_rows_per_wave != 2 && t.spmm_rows_per_wave != 4 && t.spmm_rows_per_wave != 8 &&
+        t.spmm_rows_per_wave != 16) {
+        *why = "spmm_rows_per_wave must be 2, 4, 8 or 16";
         return false;
     }
-    if (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) {
+    if (t.spmm_rows_per_wave == 16 && !d.mixed) {
+        *why = "spmm_rows_per_wave = 16 (the matrix-core SpMM) is for mixed precision";
+        return false;
+    }
+    if (t.spmm_rows_per_wave == 16) {
+        // matrix-core SpMM: 1, 2 or 4 trajectories per lane = 16, 32, 64 per wave
+        if (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2 && t.spmm_traj_per_lane != 4) {
+            *why = "with spmm_rows_per_wave = 16, spmm_traj_per_lane must be 1, 2 or 4";
+            return false;
+        }
+        if (t.spmm_phases != 1 || t.spmm_pipeline != 0) {
+            *why = "with spmm_rows_per_wave = 16, spmm_phases must be 1 and spmm_pipeline 0";
+            return false;
+        }
+    } else if (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) {
         *why = "spmm_traj_per_lane must be 1 or 2";
         return false;
     }
-    if (t.spmm_traj_per_lane == 2 && !(d.mixed && d.Bp % 128 == 0)) {
+    if (t.spmm_rows_per_wave != 16 && t.spmm_traj_per_lane == 2 && !(d.mixed && d.Bp % 128 == 0)) {
         *why = "spmm_traj_per_lane = 2 needs mixed precision and a padded batch that is a multiple of 128";
         return false;
     }
-    const int nslice = d.Bp / (64 * t.spmm_traj_per_lane);
+    const int nslice = t.spmm_rows_per_wave == 16 ? 1 : d.Bp / (64 * t.spmm_traj_per_lane);
     if ((t.spmm_phases != 1 && t.spmm_phases != 2 && t.spmm_phases != 4) || nslice % t.spmm_phases) {
         *why = "spmm_phases must be 1, 2 or 4 and divide the number of trajectory slices";
         return false;
@@ -1664,11 +1686,14 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     dq_status st = DQ_OK;
     double best_ms = 1e300;
     hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
-    for (int R : {4, 2, 8}) {
-        for (int tpl : {2, 1}) {
+    for (int R : {4, 2, 8, 16}) {
+        for (int tpl : {2, 1, 4}) {
             for (int ph : {1, 2, 4}) {
                 for (int cand = 0; cand < 8; ++cand) {
                     if (e != hipSuccess || st != DQ_OK) break;
+                    // the matrix-core SpMM has no phase or pipeline variants
+                    if (R == 16 && (ph != 1 || cand >= 4)) continue;
+                    if (R != 16 && tpl == 4) continue;
                     dq_tuning t;
                     std::memset(&t, 0, sizeof(t));
                     t.spmm_rows_per_wave = R;

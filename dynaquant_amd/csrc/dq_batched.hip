@@ -83,13 +83,13 @@ __device__ __forceinline__ double fmav(double a, double b, double c) { return __
 // costs one vector memory instruction addressed as uniform base + lane offset
 // and the vector ALU only runs the FMAs.  TPL trajectories per lane: 2 in FP32
 // (one 16-byte load per lane), 1 in FP64.
-// 8 gathers of x rows rc[p..p+7] (TPL trajectories per lane)
-template <typename VT, int TPL, typename V2>
-__device__ __forceinline__ void gather8(const V2* __restrict__ xs, const int32_t* __restrict__ rc, int p, int Bp,
-                                        int lane, VT (&xr)[8][TPL], VT (&xi)[8][TPL])
+// G gathers of x rows rc[p..p+G-1] (TPL trajectories per lane)
+template <typename VT, int TPL, int G, typename V2>
+__device__ __forceinline__ void gatherg(const V2* __restrict__ xs, const int32_t* __restrict__ rc, int p, int Bp,
+                                        int lane, VT (&xr)[G][TPL], VT (&xi)[G][TPL])
 {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < G; ++u) {
         const V2* xc = xs + (size_t)uniform(rc[p + u]) * Bp;
         if constexpr (TPL == 2) {
             const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
@@ -102,12 +102,12 @@ __device__ __forceinline__ void gather8(const V2* __restrict__ xs, const int32_t
 }
 
 // rows r of the block += value(p+u, r) * x(p+u), in column order
-template <typename VT, int TPL, int kRB>
-__device__ __forceinline__ void fma8(const VT* __restrict__ rv, int p, const VT (&xr)[8][TPL],
-                                     const VT (&xi)[8][TPL], VT (&ar)[TPL][kRB], VT (&ai)[TPL][kRB])
+template <typename VT, int TPL, int kRB, int G>
+__device__ __forceinline__ void fmag(const VT* __restrict__ rv, int p, const VT (&xr)[G][TPL],
+                                     const VT (&xi)[G][TPL], VT (&ar)[TPL][kRB], VT (&ai)[TPL][kRB])
 {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < G; ++u) {
 #pragma unroll
         for (int r = 0; r < kRB; ++r) {
             const VT v = rv[(size_t)(p + u) * kRB + r];
@@ -158,29 +158,54 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
             for (int r = 0; r < kRB; ++r) { ar[t][r] = 0; ai[t][r] = 0; }
         int p = u0;
         if constexpr (PIPE) {
-            // two register buffers: the gathers of the next 8 columns are in
-            // flight while the current 8 are multiplied in
-            const int nb = (u1 - p) / 8;
-            VT xar[8][TPL], xai[8][TPL], xbr[8][TPL], xbi[8][TPL];
-            if (nb > 0) gather8<VT, TPL>(xs, rc, p, Bp, lane, xar, xai);
+            // two 4-column register buffers (the registers of one 8-column
+            // buffer): the gathers of the next 4 columns are in flight while
+            // the current 4 are multiplied in
+            const int nb = (u1 - p) / 4;
+            VT xar[4][TPL], xai[4][TPL], xbr[4][TPL], xbi[4][TPL];
+            if (nb > 0) gatherg<VT, TPL, 4>(xs, rc, p, Bp, lane, xar, xai);
             int bb = 0;
             for (; bb + 2 <= nb; bb += 2) {
-                gather8<VT, TPL>(xs, rc, p + 8, Bp, lane, xbr, xbi);
-                fma8<VT, TPL, kRB>(rv, p, xar, xai, ar, ai);
-                p += 8;
-                if (bb + 2 < nb) gather8<VT, TPL>(xs, rc, p + 8, Bp, lane, xar, xai);
-                fma8<VT, TPL, kRB>(rv, p, xbr, xbi, ar, ai);
-                p += 8;
+                gatherg<VT, TPL, 4>(xs, rc, p + 4, Bp, lane, xbr, xbi);
+                fmag<VT, TPL, kRB, 4>(rv, p, xar, xai, ar, ai);
+                p += 4;
+                if (bb + 2 < nb) gatherg<VT, TPL, 4>(xs, rc, p + 4, Bp, lane, xar, xai);
+                fmag<VT, TPL, kRB, 4>(rv, p, xbr, xbi, ar, ai);
+                p += 4;
             }
             if (bb < nb) {
-                fma8<VT, TPL, kRB>(rv, p, xar, xai, ar, ai);
-                p += 8;
+                fmag<VT, TPL, kRB, 4>(rv, p, xar, xai, ar, ai);
+                p += 4;
             }
         } else {
+            // the column offsets of the next 8 columns are loaded while this
+            // group's gathers are in flight, so a scalar-load round trip no
+            // longer precedes every group of gathers
+            int cur[8], nxt[8];
+            if (p + 8 <= u1) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cur[u] = uniform(rc[p + u]);
+            }
             for (; p + 8 <= u1; p += 8) {
                 VT xr[8][TPL], xi[8][TPL];
-                gather8<VT, TPL>(xs, rc, p, Bp, lane, xr, xi);
-                fma8<VT, TPL, kRB>(rv, p, xr, xi, ar, ai);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const V2* xc = xs + (size_t)cur[u] * Bp;
+                    if constexpr (TPL == 2) {
+                        const float4 q = *reinterpret_cast<const float4*>(xc + 2 * lane);
+                        xr[u][0] = q.x; xi[u][0] = q.y; xr[u][1] = q.z; xi[u][1] = q.w;
+                    } else {
+                        const V2 q = xc[lane];
+                        xr[u][0] = q.x; xi[u][0] = q.y;
+                    }
+                }
+                if (p + 16 <= u1) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) nxt[u] = uniform(rc[p + 8 + u]);
+                }
+                fmag<VT, TPL, kRB, 8>(rv, p, xr, xi, ar, ai);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) cur[u] = nxt[u];
             }
         }
         for (; p < u1; ++p) {
@@ -263,8 +288,200 @@ static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, i
     }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision SpMM on the matrix cores.  A block of 16 rows is a dense
+// 16 x U matrix over the union of its columns (U padded to a multiple of 16
+// with zero columns), so the block's rows of w are that matrix times the U
+// gathered rows of x: for a slice of 8 NG trajectories a (16 x U) x (U x 16 NG)
+// FP32 product, the 16 NG columns being re/im of the trajectories.  One wave
+// per row block and slice; a K-step of 4 union columns is NG
+// v_mfma_f32_16x16x4_f32.  MFMA lanes: A[i][k] from lane i + 16 k, B[k][c]
+// from lane c + 16 k, D[4 q + r][c] in register r of lane c + 16 q.  Group g
+// holds the real columns NG c + g, so a lane loads NG contiguous floats of one
+// x row per step and ends with rows 4 q .. 4 q + 3 of the trajectories
+// NG c / 2 .. NG c / 2 + NG / 2 - 1 of its slice.  The loads run kDepth steps
+// ahead of the multiplies (column offset, value and x row of a step are
+// refilled as soon as the step is consumed).  Each output element is the same
+// sequence of MFMA accumulations whatever NG, so the slice width does not
+// change a bit of the result.  Against the 4-row VALU kernel: half the
+// gathered bytes per row at 2.7x the multiply-adds, which run on the matrix
+// cores.  On the 12,000-site dipole tube (B = 256) it runs level with that
+// kernel (42 us); it is an autotuner choice for denser couplings.
+// ---------------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <int NG> struct XV;
+template <> struct XV<2> { using T = float2; };
+template <> struct XV<4> { using T = float4; };
+template <> struct XV<8> { struct T { float4 a, b; }; };
+
+template <int NG>
+__device__ __forceinline__ typename XV<NG>::T load_xv(const float* p)
+{
+    if constexpr (NG == 2) return *reinterpret_cast<const float2*>(p);
+    else if constexpr (NG == 4) return *reinterpret_cast<const float4*>(p);
+    else return typename XV<8>::T{*reinterpret_cast<const float4*>(p), *reinterpret_cast<const float4*>(p + 4)};
+}
+
+template <int NG>
+__device__ __forceinline__ float xv_at(const typename XV<NG>::T& v, int g)
+{
+    if constexpr (NG == 2) return g == 0 ? v.x : v.y;
+    else if constexpr (NG == 4) return g == 0 ? v.x : (g == 1 ? v.y : (g == 2 ? v.z : v.w));
+    else {
+        const float4& h = g < 4 ? v.a : v.b;
+        const int q = g & 3;
+        return q == 0 ? h.x : (q == 1 ? h.y : (q == 2 ? h.z : h.w));
+    }
+}
+
+template <int NG>
+__global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
+{
+    constexpr int TW = 8 * NG;                       // trajectories per wave
+    constexpr int TL = NG / 2;                       // trajectories per lane
+    constexpr int kDepth = 4;
+    using XT = typename XV<NG>::T;
+    const int ntt = d.sn / TW;
+    const int logical = xcd_tile(blockIdx.x, gridDim.x);
+    if (logical >= d.ntile_s * ntt) return;
+    // trajectory-slice-major: an XCD walks all rows of one or two slices, so the
+    // slice's x rows (n x 8 NG x 8 B) stay in its L2 across their ~11 reuses
+    const int rtile = logical % d.ntile_s, ttile = d.s0 / TW + logical / d.ntile_s;
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int kq = lane >> 4, nc = lane & 15;
+    const int Bp = d.Bp;
+    const float2* __restrict__ x = reinterpret_cast<const float2*>(d.V) + (size_t)j * d.vs;
+    float2* __restrict__ y = reinterpret_cast<float2*>(d.V) + (size_t)(j + 1) * d.vs;
+    const float* __restrict__ xf = reinterpret_cast<const float*>(x) + (size_t)ttile * 2 * TW + NG * nc;
+    const float* __restrict__ rv = d.rb_val32;
+    const int32_t* __restrict__ rc = d.rb_col;
+    const size_t rowf = (size_t)2 * Bp;              // floats per row of x
+    const int nblk = (d.n + 15) / 16;
+    const int rb = rtile * 4 + wave;
+    const int b0 = ttile * TW + TL * nc;             // first of this lane's trajectories
+    double acc[TL][2];
+#pragma unroll
+    for (int t = 0; t < TL; ++t) { acc[t][0] = 0.0; acc[t][1] = 0.0; }
+    const int slice_nact = uniform(d.nact[ttile * TW]);
+    if (rb < nblk && rb * 16 < slice_nact) {
+        const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+        mfma_f4 c[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        // column offsets come through the scalar cache (four per step, wave-
+        // uniform): a vector load there would make every gather wait for all
+        // the loads in flight before it.  The union is padded to a multiple of
+        // 4 kDepth columns, so every refill is issued (past the end it reloads
+        // a valid step) and the waits on the loads in flight are static.
+        auto cols4 = [&](int p) {
+            const int4 q4 = *reinterpret_cast<const int4*>(rc + p);
+            return make_int4(uniform(q4.x), uniform(q4.y), uniform(q4.z), uniform(q4.w));
+        };
+        auto pick = [&](const int4& q4) { return kq == 0 ? q4.x : (kq == 1 ? q4.y : (kq == 2 ? q4.z : q4.w)); };
+        if (u1 > u0) {
+            float av[kDepth];
+            XT xv[kDepth];
+#pragma unroll
+            for (int q = 0; q < kDepth; ++q) {
+                const int p = u0 + 4 * q;
+                av[q] = rv[(size_t)(p + kq) * 16 + nc];
+                xv[q] = load_xv<NG>(xf + (size_t)pick(cols4(p)) * rowf);
+            }
+            int pnext = u0 + 4 * kDepth < u1 ? u0 + 4 * kDepth : u0;
+            int4 cn = cols4(pnext);
+            for (int base = u0; base < u1; base += 4 * kDepth) {
+#pragma unroll
+                for (int q = 0; q < kDepth; ++q) {
+#pragma unroll
+                    for (int g = 0; g < NG; ++g)
+                        c[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], xv_at<NG>(xv[q], g), c[g], 0, 0, 0);
+                    av[q] = rv[(size_t)(pnext + kq) * 16 + nc];
+                    xv[q] = load_xv<NG>(xf + (size_t)pick(cn) * rowf);
+                    const int pn = base + 4 * q + 4 * kDepth + 4;
+                    pnext = pn < u1 ? pn : u0;
+                    cn = cols4(pnext);
+                }
+            }
+        }
+        const float g = (float)d.g;
+        const float* __restrict__ diag = reinterpret_cast<const float*>(d.diag);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = rb * 16 + 4 * kq + i;
+            if (r < d.n) {
+                const size_t e = (size_t)r * Bp + b0;
+                float xs[2 * TL], dv[TL], w[2 * TL];
+#pragma unroll
+                for (int t = 0; t < TL; ++t) {
+                    const float2 q = x[e + t];
+                    xs[2 * t] = q.x;
+                    xs[2 * t + 1] = q.y;
+                    dv[t] = diag[e + t];
+                }
+#pragma unroll
+                for (int t = 0; t < TL; ++t) {
+                    float wr = fmav(g, c[2 * t][i], dv[t] * xs[2 * t]);
+                    float wi = fmav(g, c[2 * t + 1][i], dv[t] * xs[2 * t + 1]);
+                    if (r >= d.nact[b0 + t]) { wr = 0.f; wi = 0.f; }
+                    w[2 * t] = wr;
+                    w[2 * t + 1] = wi;
+                    const double xr = (double)xs[2 * t], xm = (double)xs[2 * t + 1], dr = (double)wr, dm = (double)wi;
+                    if (!last) {
+                        acc[t][0] = __builtin_fma(xm, dm, __builtin_fma(xr, dr, acc[t][0]));
+                        acc[t][1] = __builtin_fma(-xm, dr, __builtin_fma(xr, dm, acc[t][1]));
+                    } else {
+                        acc[t][0] = __builtin_fma(dm, dm, __builtin_fma(dr, dr, acc[t][0]));
+                    }
+                }
+                if (!last) {
+#pragma unroll
+                    for (int t = 0; t < TL; ++t) y[e + t] = make_float2(w[2 * t], w[2 * t + 1]);
+                }
+            }
+        }
+    }
+    // rows of the block: the four lane groups kq, then the four waves
+#pragma unroll
+    for (int t = 0; t < TL; ++t)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            acc[t][q] += __shfl_xor(acc[t][q], 16, 64);
+            acc[t][q] += __shfl_xor(acc[t][q], 32, 64);
+        }
+    __shared__ double lds[4][2][TL][16];
+    if (kq == 0) {
+#pragma unroll
+        for (int t = 0; t < TL; ++t) { lds[wave][0][t][nc] = acc[t][0]; lds[wave][1][t][nc] = acc[t][1]; }
+    }
+    __syncthreads();
+    if (wave == 0 && kq == 0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (q == 1 && last) break;
+#pragma unroll
+            for (int t = 0; t < TL; ++t) {
+                const double v = (lds[0][q][t][nc] + lds[1][q][t][nc]) + (lds[2][q][t][nc] + lds[3][q][t][nc]);
+                d.part[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
+            }
+        }
+    }
+}
+
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
+    if (d.rb == 16) {
+        // the matrix-core kernel (mixed precision only, enforced by the tuning
+        // checks); spmm_traj_per_lane 1, 2, 4 = 16, 32, 64 trajectories per wave
+        const int tl = d.spmm_tpl;
+        const int tw = 16 * tl;
+        const dim3 grid(bpad8(d.ntile_s * (d.sn / tw))), block(kThreads);
+        if (tl == 4) hipLaunchKernelGGL((k_spmm_mfma<8>), grid, block, 0, s, d, j, last);
+        else if (tl == 2) hipLaunchKernelGGL((k_spmm_mfma<4>), grid, block, 0, s, d, j, last);
+        else hipLaunchKernelGGL((k_spmm_mfma<2>), grid, block, 0, s, d, j, last);
+        return hipGetLastError();
+    }
     // trajectory slices of 64 * TPL; the slices run as `phases` launches of
     // ntt slices each, so that one launch's gathered rows fit the L2 of an XCD
     const int tpl = (d.mixed && d.spmm_tpl == 2 && d.sn % 128 == 0) ? 2 : 1;
@@ -318,8 +535,8 @@ __global__ __launch_bounds__(kThreads) void k_tk_power_rb(Dev d, int k)
         int p = u0;
         for (; p + 8 <= u1; p += 8) {
             double xr[8][1], xi[8][1];
-            gather8<double, 1>(xs, rc, p, Bp, lane, xr, xi);
-            fma8<double, 1, kRB>(rv, p, xr, xi, ar, ai);
+            gatherg<double, 1, 8>(xs, rc, p, Bp, lane, xr, xi);
+            fmag<double, 1, kRB, 8>(rv, p, xr, xi, ar, ai);
         }
         for (; p < u1; ++p) {
             const double2 q = xs[(size_t)uniform(rc[p]) * Bp + lane];
@@ -369,6 +586,7 @@ hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s)
     switch (d.rb) {
     case 2: hipLaunchKernelGGL((k_tk_power_rb<2>), grid, block, 0, s, d, k); break;
     case 8: hipLaunchKernelGGL((k_tk_power_rb<8>), grid, block, 0, s, d, k); break;
+    case 16: hipLaunchKernelGGL((k_tk_power_rb<16>), grid, block, 0, s, d, k); break;
     default: hipLaunchKernelGGL((k_tk_power_rb<4>), grid, block, 0, s, d, k); break;
     }
     return hipGetLastError();

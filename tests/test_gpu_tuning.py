@@ -25,6 +25,8 @@ def _run(engine, sysd, precision, B, tuning, steps=40):
 
 SPMM_CHOICES = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=p, spmm_pipeline=q)
                 for r in (2, 4, 8) for t in (1, 2) for p in (1, 2) for q in (0, 1)]
+# the matrix-core SpMM (16-row blocks): 16, 32 or 64 trajectories per wave
+SPMM_CHOICES += [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0) for t in (1, 2, 4)]
 
 
 def test_spmm_choices_mixed(engine):
@@ -45,7 +47,7 @@ def test_spmm_choices_mixed(engine):
         print(c, np.abs(pops - ref[r][0]).max())
         assert np.array_equal(pops, ref[r][0]), c
         assert np.array_equal(psi, ref[r][1]), c
-    for r in (2, 8):
+    for r in (2, 8, 16):
         d = np.abs(ref[r][0] - ref[4][0]).max()
         print("rows per wave", r, "vs 4:", d)
         assert d < 2e-6
@@ -74,7 +76,8 @@ def test_invalid_tuning_rejected(engine):
     sysd = ring_tube(engine)
     s = make_engine(engine, sysd, 16, 1.0, B=64, force_batched=True, precision=engine.FP64)
     for bad in (dict(spmm_rows_per_wave=3), dict(spmm_traj_per_lane=2), dict(spmm_phases=3),
-                dict(tile_rows=100), dict(concurrent_slices=3), dict(concurrent_slices=2)):
+                dict(tile_rows=100), dict(concurrent_slices=3), dict(concurrent_slices=2),
+                dict(spmm_rows_per_wave=16)):
         with pytest.raises(engine.DQError):
             s.set_tuning(**bad)
     s.close()
