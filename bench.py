@@ -74,26 +74,41 @@ def update_bytes(n, b, m, mixed):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_N12000_B256_m24.txt")
 
 
+def spmm_kernel_name(tuning):
+    """the SpMM kernel a tuning launches (mixed precision)"""
+    if tuning["spmm_rows_per_wave"] == 16:
+        return "dq::k_spmm_mfma<%d>" % (2 * tuning["spmm_traj_per_lane"])
+    return "dq::k_spmm_b<float,%d,%d,%s>" % (tuning["spmm_traj_per_lane"], tuning["spmm_rows_per_wave"],
+                                             "true" if tuning["spmm_pipeline"] else "false")
+
+
+def update_kernel_name(tuning):
+    return "dq::k_update_b<float,KW> tile_rows=%d" % tuning["tile_rows"]
+
+
 def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` ("update" or "spmm") from the committed
+    """HBM bytes per launch of `kernel` (the names above) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (already scaled by the copy-kernel
-    factor recorded there); None when that summary is absent"""
+    factor recorded there); None when that summary has no line for it"""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                key, _, val = line.partition("=")
-                if key.strip() == "traffic_bytes_per_launch." + kernel:
+                key, _, val = line.rpartition("=")
+                if key.strip() == "traffic_bytes_per_launch " + kernel:
                     return float(val)
     except OSError:
         pass
     return None
 
 
-def traffic_probe(dq, t, steps=2):
+def traffic_probe(dq, t, tuning, steps=2):
     """workload for the rocprofv3 --pmc runs: a few batched steps without the
     graph, then one 256 MiB device copy whose counters give the scale factor"""
     import ctypes
     s = make_system(dq, t, B, dq.MIXED, steps + 1)
+    if tuning:
+        s.set_tuning(**tuning)
+    print("# traffic probe launch choices: %s" % s.tuning(), file=sys.stderr, flush=True)
     s.run(steps, graph=False)
     s.close()
     hip = ctypes.CDLL("libamdhip64.so")
@@ -149,6 +164,8 @@ def main():
                          "averages would otherwise include the autotuner's trial launches)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only the short workload profiled for HBM traffic counters")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="launch choices as JSON (dq_tuning field names) instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; trajectories shard across GPUs with no "
@@ -160,15 +177,21 @@ def main():
     n, nnz = t["n"], len(t["col"])
     K, W = args.steps, args.warmup
     print("# workload N=%d nnz_off=%d B=%d m=%d dt=%g" % (n, nnz, B, M, DT), file=sys.stderr, flush=True)
+    fixed = json.loads(args.tuning) if args.tuning else None
     if args.traffic_probe:
-        traffic_probe(dq, t)
+        traffic_probe(dq, t, fixed)
         return
 
     # ---- headline: batched mixed precision, hipGraph per step
     s = make_system(dq, t, B, dq.MIXED, W + K + args.profile_steps)
-    tuning = s.tuning() if args.no_autotune else s.autotune(reps=2)
-    print("# launch choices (%s): %s" % ("default" if args.no_autotune else "autotuned", tuning),
-          file=sys.stderr, flush=True)
+    if fixed:
+        s.set_tuning(**fixed)
+        tuning, how = s.tuning(), "given"
+    elif args.no_autotune:
+        tuning, how = s.tuning(), "default"
+    else:
+        tuning, how = s.autotune(reps=2), "autotuned"
+    print("# launch choices (%s): %s" % (how, tuning), file=sys.stderr, flush=True)
     s.run(W, graph=True)
     s.run(K, graph=True)
     ms = s.last_run_ms()
@@ -193,14 +216,14 @@ def main():
     dominant = max(totals, key=totals.get)
     print("# kernel ms per step: " + ", ".join("%s %.3f" % (k, v / cyc) for k, v in totals.items()),
           file=sys.stderr, flush=True)
+    upd_name, spmm_name = update_kernel_name(tuning), spmm_kernel_name(tuning)
     upd_roof = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("update"),
-                "kernel": "dq::k_update_b<float,KW> (KW = 2/4/6 for j+1 <= 8/16/24)",
+                "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(upd_name),
+                "kernel": upd_name + " (KW = 2/4/6 for j+1 <= 8/16/24)",
                 "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms}
     spmm_roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic("spmm"),
-                 "kernel": "dq::k_spmm_b<float,2>", "bytes_per_launch": b_spmm,
-                 "avg_launch_ms": spmm_ms}
+                 "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(spmm_name),
+                 "kernel": spmm_name, "bytes_per_launch": b_spmm, "avg_launch_ms": spmm_ms}
     roof = spmm_roof if dominant == "spmm" else upd_roof
 
     # ---- plain FP64 single-trajectory path, same tube

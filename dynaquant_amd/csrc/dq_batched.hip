@@ -345,9 +345,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
     const int ntt = d.sn / TW;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     if (logical >= d.ntile_s * ntt) return;
-    // trajectory-slice-major: an XCD walks all rows of one or two slices, so the
-    // slice's x rows (n x 8 NG x 8 B) stay in its L2 across their ~11 reuses
-    const int rtile = logical % d.ntile_s, ttile = d.s0 / TW + logical / d.ntile_s;
+    // row-tile-major: the slices of a row block run on one XCD, so the block's
+    // dense values (2.7x the nonzeros) come from HBM once per launch
+    const int rtile = logical / ntt, ttile = d.s0 / TW + logical % ntt;
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int kq = lane >> 4, nc = lane & 15;
