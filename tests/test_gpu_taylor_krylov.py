@@ -80,6 +80,33 @@ def test_taylor_krylov_launch_variants_bitwise(engine):
     assert np.array_equal(ea, eb) and np.array_equal(ea, ec)
 
 
+def test_taylor_krylov_with_matrix_core_spmm(engine):
+    """the auxiliary Lanczos on the matrix-core SpMM and the 16-row Taylor
+    powers: mixed B = 128 against the FP64 path, like the 4-row kernels.
+    dt = 0.5 fs: at 2 fs (tau |A| / hbar ~ 19 on this tube) the Taylor part's
+    cancellations amplify FP32 rounding to ~1e-4 for every SpMM variant"""
+    sysd = ring_tube(engine)
+    n, steps = 600, 80
+    rp, col, val, adiag = aux_matrix(sysd)
+    psi0 = np.zeros(n, complex)
+    psi0[100] = 1.0
+    out = {}
+    for key, prec, tun in (("fp64", engine.FP64, dict(spmm_rows_per_wave=4)),
+                           ("rows4", engine.MIXED, dict(spmm_rows_per_wave=4)),
+                           ("rows16", engine.MIXED, dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2))):
+        s = make_engine(engine, sysd, 24, 0.5, B=128, precision=prec, force_batched=True)
+        s.set_tuning(**tun)
+        s.set_taylor_krylov(10, 24, rp, col, val, adiag)
+        s.set_state(psi0)
+        s.run(steps, graph=True)
+        out[key] = np.abs(s.get_state()) ** 2
+        s.close()
+    for key in ("rows4", "rows16"):
+        d = np.abs(out[key] - out["fp64"]).max()
+        print("Taylor-Krylov mixed %s vs FP64: max site-population error %.3e" % (key, d))
+        assert d < 1e-6
+
+
 def test_taylor_krylov_switch_and_errors(engine):
     sysd = ring_tube(engine)
     n = 600

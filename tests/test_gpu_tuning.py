@@ -139,3 +139,35 @@ def test_concurrent_slices_bitwise_equal(engine):
         s.close()
     for o in out[1:]:
         assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+
+
+def test_matrix_core_spmm_ragged_and_sliced(engine):
+    """the matrix-core SpMM (16-row blocks) on a ragged mixed batch: close to
+    the 4-row VALU SpMM, rows beyond n_active stay 0, and two concurrent
+    slices are bitwise the single-stream run"""
+    sysd = ring_tube(engine)
+    n = sysd["n"]
+    na = np.array([600, 136, 488, 360] * 32, np.int32)     # not multiples of 16
+    psi0 = np.zeros((128, n), complex)
+    psi0[:, 0] = 1.0
+    grp = (np.arange(n) // 150).astype(np.int32)
+    out = {}
+    for key, tun, graph in (("valu", dict(spmm_rows_per_wave=4), True),
+                            ("mfma", dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2), True),
+                            ("mfma2", dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2, concurrent_slices=2), True),
+                            ("mfma_eager", dict(spmm_rows_per_wave=16, spmm_traj_per_lane=4), False)):
+        s = make_engine(engine, sysd, 16, 1.0, B=128, precision=engine.MIXED, n_active=na)
+        s.set_tuning(**tun)
+        s.set_state(psi0)
+        s.set_observables(grp, 4, 5, 40)
+        s.run(40, graph=graph)
+        out[key] = (s.populations(), s.get_state())
+        s.close()
+    d = np.abs(out["mfma"][0] - out["valu"][0]).max()
+    print("ragged mixed: matrix-core vs VALU SpMM max group-population difference %.3e" % d)
+    assert d < 2e-6
+    for key in ("mfma2", "mfma_eager"):
+        assert np.array_equal(out[key][0], out["mfma"][0]) and np.array_equal(out[key][1], out["mfma"][1]), key
+    st = out["mfma"][1]
+    for b in range(128):
+        assert np.all(st[b, na[b]:] == 0)
