@@ -16,11 +16,11 @@ def aux_matrix(sysd):
     return sysd["rp"], sysd["col"], sysd["val"], np.linspace(-40.0, 40.0, n)
 
 
-@pytest.mark.parametrize("B,batched", [(1, False), (4, True)])
-def test_taylor_krylov_matches_oracle(engine, oracle, B, batched):
+@pytest.mark.parametrize("B,batched,K", [(1, False, 1), (4, True, 1), (2, True, 2)])
+def test_taylor_krylov_matches_oracle(engine, oracle, B, batched, K):
     """N = 600 tube with OU noise and static disorder, p = 6, m_aux = 16,
-    200 steps; site populations every 10 steps, C(n) and the error sums
-    (ragged n_active in the batched case)"""
+    200 steps of K Taylor-Krylov cycles; site populations every 10 steps,
+    C(n) and the error sums (ragged n_active in the batched cases)"""
     sysd = ring_tube(engine)
     n, steps = 600, 200
     na = [600, 360, 480, 240][:B] if batched else None
@@ -28,22 +28,22 @@ def test_taylor_krylov_matches_oracle(engine, oracle, B, batched):
     psi0 = np.zeros((B, n), complex)
     for b in range(B):
         psi0[b, [5, 66]] = np.sqrt([0.7, 0.3])
-    s = make_engine(engine, sysd, 24, 1.0, B=B, force_batched=batched, n_active=na)
+    s = make_engine(engine, sysd, 24, 1.0, B=B, K=K, force_batched=batched, n_active=na)
     s.set_taylor_krylov(6, 16, rp, col, val, adiag)
     s.set_state(psi0)
     grp = np.arange(n, dtype=np.int32)
     s.set_observables(grp, n, 10, steps, bra=psi0[0])
     s.run(steps, graph=True)
     pops, C, err = s.populations(), s.corr(), s.error_sum()
-    p, sites = make_oracle(oracle, sysd, 24, 1.0)
+    p, sites = make_oracle(oracle, sysd, 24, 1.0, K=K)
     keep = oracle.set_taylor_krylov(p, 6, 16, rp, col, val, adiag)
     rb = oracle.run_batch(p, sites, sysd["rp"], sysd["col"], sysd["val"], psi0, list(range(B)), steps, 10,
                           grp, n, n_active=na, bra=psi0[0])
     perr = np.abs(pops - rb["pops"]).max()
     cerr = np.abs(C - rb["corr"]).max()
     eerr = np.abs(err - rb["err"]).max() / np.abs(rb["err"]).max()
-    print("Taylor-Krylov B=%d: max |P - P_oracle| %.3e, max |C - C_oracle| %.3e, error-sum rel %.3e (sum %.3e)"
-          % (B, perr, cerr, eerr, rb["err"].max()))
+    print("Taylor-Krylov B=%d K=%d: max |P - P_oracle| %.3e, max |C - C_oracle| %.3e, error-sum rel %.3e (sum %.3e)"
+          % (B, K, perr, cerr, eerr, rb["err"].max()))
     assert perr <= 1e-10 and cerr <= 1e-10
     assert eerr <= 1e-6
     if batched:
