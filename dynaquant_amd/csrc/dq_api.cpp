@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1685,6 +1686,7 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     dq_tuning orig = best;
     dq_status st = DQ_OK;
     double best_ms = 1e300;
+    std::vector<std::pair<float, dq_tuning>> timed_choices;
     hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
     for (int R : {4, 2, 8, 16}) {
         for (int tpl : {2, 1, 4}) {
@@ -1716,17 +1718,27 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                         if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
                         ms_min = std::min(ms_min, ms);
                     }
+                    if (e == hipSuccess) timed_choices.push_back({ms_min, t});
                     if (e == hipSuccess && ms_min < best_ms) { best_ms = ms_min; best = t; }
                 }
             }
         }
     }
-    // concurrent slices: the chosen kernels, captured as graphs of 1, 2, 4 slices
+    // concurrent slices: the four fastest kernel choices of one stream, each
+    // captured as graphs of 1, 2 and 4 slices (a choice that is fastest alone
+    // is not always the one that overlaps best with the other slices)
     if (e == hipSuccess && st == DQ_OK) {
-        if ((st = apply_tuning(sys, best)) == DQ_OK) {
-            double best_ms = 1e300;
+        std::stable_sort(timed_choices.begin(), timed_choices.end(),
+                         [](const std::pair<float, dq_tuning>& a, const std::pair<float, dq_tuning>& b) {
+                             return a.first < b.first;
+                         });
+        if (timed_choices.size() > 4) timed_choices.resize(4);
+        double best_graph_ms = 1e300;
+        for (const auto& cand : timed_choices) {
+            if (e != hipSuccess || st != DQ_OK) break;
+            if ((st = apply_tuning(sys, cand.second)) != DQ_OK) break;
             for (int ns : {1, 2, 4}) {
-                dq_tuning t = best;
+                dq_tuning t = cand.second;
                 t.concurrent_slices = ns;
                 std::string why;
                 if (!tuning_valid(sys, t, &why)) continue;
@@ -1734,7 +1746,8 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 if ((e = capture_tuning_slices(sys, ns, &ge)) != hipSuccess) break;
                 float ms_min = 1e30f;
                 e = hipGraphLaunch(ge, sys->stream);                            // warm-up
-                for (int r = 0; r < reps && e == hipSuccess; ++r) {
+                // the candidates differ by a few percent: at least 5 timed replays
+                for (int r = 0; r < std::max(reps, 5) && e == hipSuccess; ++r) {
                     float ms = 0.f;
                     if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
                     if ((e = hipGraphLaunch(ge, sys->stream)) != hipSuccess) break;
@@ -1745,7 +1758,7 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 }
                 hipGraphExecDestroy(ge);
                 if (e != hipSuccess) break;
-                if (ms_min < best_ms) { best_ms = ms_min; best.concurrent_slices = ns; }
+                if (ms_min < best_graph_ms) { best_graph_ms = ms_min; best = t; }
             }
         }
     }
