@@ -145,9 +145,36 @@ def cpu_baseline(t, seconds_target=15.0):
     O.run_batch(p, sites, t["rp"], t["col"], t["val"], t["psi0"], np.arange(cores), steps2, EVERY,
                 t["groups"], 8, nthreads=cores)
     dt = time.perf_counter() - t0
+    # single core: the N = 12,000 workload and the 600-site single-wall tube (m = 16, dt = 1 fs)
+    t1 = time.perf_counter()
+    O.run_batch(p, sites, t["rp"], t["col"], t["val"], t["psi0"], np.arange(1), 10, EVERY, t["groups"], 8,
+                nthreads=1)
+    single12k = (time.perf_counter() - t1) / 10 * 1e3
+    import dynaquant_amd as dq
+    pos, dip, wall, rp6, col6, val6 = dq.tube([10], [3.0])
+    p6 = O.make_params(600, 16, 1.0, reorth=O.REORTH_GRAM, sigma_static=200.0, seed=1)
+    s6 = O.Sites(600, tau=TAU, sigma=300.0, e_site=0.0)
+    psi6 = np.zeros(600, complex)
+    psi6[0] = 1.0
+    t1 = time.perf_counter()
+    O.run_batch(p6, s6, rp6, col6, val6, psi6, np.arange(1), 200, 10, np.arange(600, dtype=np.int32), 600,
+                nthreads=1)
+    single600 = (time.perf_counter() - t1) / 200 * 1e3
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": cores * steps2 / dt, "unit": "traj-steps/s", "cores": cores, "kind": "port",
             "sample": "%d trajectories x %d steps of the same N=12000 m=24 dt=2fs workload, FP64 "
-                      "oracle, one trajectory per OpenMP thread, %.1f s" % (cores, steps2, dt)}
+                      "oracle, one trajectory per OpenMP thread, %.1f s" % (cores, steps2, dt),
+            "cpu_model": model, "host_cpus": os.cpu_count(),
+            "single_core_ms_per_traj_step": {"N=12000 m=24 dt=2fs": single12k,
+                                             "N=600 m=16 dt=1fs": single600}}
 
 
 def main():
