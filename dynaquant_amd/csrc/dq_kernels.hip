@@ -642,28 +642,55 @@ __device__ __forceinline__ double tbar_at(int i, int jj, int m, int aug, const d
     return 0.0;
 }
 
+// Products of n x n matrices in LDS, a 2 x 2 block of C per thread (one pass
+// for n <= 32: half the LDS reads of one element per thread).  Each element
+// is the same k-ordered sum as a one-element-per-thread product.
 __device__ void mm_real(int n, const double* A, const double* B, double* C)
 {
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e / n, jj = e % n;
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) s += A[i * n + k] * B[k * n + jj];
-        C[e] = s;
+    const int nb = (n + 1) >> 1;
+    for (int blk = threadIdx.x; blk < nb * nb; blk += blockDim.x) {
+        const int i0 = 2 * (blk / nb), j0 = 2 * (blk % nb);
+        const int i1 = i0 + 1 < n ? i0 + 1 : i0, j1 = j0 + 1 < n ? j0 + 1 : j0;
+        double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double a0 = A[i0 * n + k], a1 = A[i1 * n + k];
+            const double b0 = B[k * n + j0], b1 = B[k * n + j1];
+            s00 += a0 * b0;
+            s01 += a0 * b1;
+            s10 += a1 * b0;
+            s11 += a1 * b1;
+        }
+        C[i0 * n + j0] = s00;
+        if (j0 + 1 < n) C[i0 * n + j0 + 1] = s01;
+        if (i0 + 1 < n) C[(i0 + 1) * n + j0] = s10;
+        if (i0 + 1 < n && j0 + 1 < n) C[(i0 + 1) * n + j0 + 1] = s11;
     }
     __syncthreads();
 }
 
 __device__ void mm_cplx(int n, const double2* A, const double2* B, double2* C)
 {
-    for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-        const int i = e / n, jj = e % n;
-        double sr = 0.0, si = 0.0;
+    const int nb = (n + 1) >> 1;
+    for (int blk = threadIdx.x; blk < nb * nb; blk += blockDim.x) {
+        const int i0 = 2 * (blk / nb), j0 = 2 * (blk % nb);
+        const int i1 = i0 + 1 < n ? i0 + 1 : i0, j1 = j0 + 1 < n ? j0 + 1 : j0;
+        double r00 = 0.0, q00 = 0.0, r01 = 0.0, q01 = 0.0, r10 = 0.0, q10 = 0.0, r11 = 0.0, q11 = 0.0;
         for (int k = 0; k < n; ++k) {
-            const double2 a = A[i * n + k], bb = B[k * n + jj];
-            sr += a.x * bb.x - a.y * bb.y;
-            si += a.x * bb.y + a.y * bb.x;
+            const double2 a0 = A[i0 * n + k], a1 = A[i1 * n + k];
+            const double2 b0 = B[k * n + j0], b1 = B[k * n + j1];
+            r00 += a0.x * b0.x - a0.y * b0.y;
+            q00 += a0.x * b0.y + a0.y * b0.x;
+            r01 += a0.x * b1.x - a0.y * b1.y;
+            q01 += a0.x * b1.y + a0.y * b1.x;
+            r10 += a1.x * b0.x - a1.y * b0.y;
+            q10 += a1.x * b0.y + a1.y * b0.x;
+            r11 += a1.x * b1.x - a1.y * b1.y;
+            q11 += a1.x * b1.y + a1.y * b1.x;
         }
-        C[e] = make_double2(sr, si);
+        C[i0 * n + j0] = make_double2(r00, q00);
+        if (j0 + 1 < n) C[i0 * n + j0 + 1] = make_double2(r01, q01);
+        if (i0 + 1 < n) C[(i0 + 1) * n + j0] = make_double2(r10, q10);
+        if (i0 + 1 < n && j0 + 1 < n) C[(i0 + 1) * n + j0 + 1] = make_double2(r11, q11);
     }
     __syncthreads();
 }
@@ -683,17 +710,20 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     __shared__ int s_piv;
     __shared__ double2 s_f[kNMax];
     const double cim = -theta;            // A = i cim Tbar  (propagate.cu:91-92)
-    // infinity norm: row sums of |A_ij| in column order, as the oracle does
-    if (threadIdx.x == 0) {
-        double nrm = 0.0;
-        for (int i = 0; i < n; ++i) {
-            double rs = 0.0;
+    // infinity norm: row sums of |A_ij| in column order, as the oracle does,
+    // one row per lane of wave 0, then the (exact) maximum over the rows
+    if (threadIdx.x < 64) {
+        const int i = threadIdx.x;
+        double rs = 0.0;
+        if (i < n)
             for (int jj = 0; jj < n; ++jj) rs += __builtin_fabs(cim * tbar_at(i, jj, m, aug, al, be));
-            if (rs > nrm) nrm = rs;
+        for (int off = 32; off > 0; off >>= 1) rs = __builtin_fmax(rs, __shfl_xor(rs, off, 64));
+        if (threadIdx.x == 0) {
+            double nrm = rs;
+            int s = 0;
+            while (nrm > 0.5) { nrm = 0.5 * nrm; s = s + 1; }
+            s_scale = s;
         }
-        int s = 0;
-        while (nrm > 0.5) { nrm = 0.5 * nrm; s = s + 1; }
-        s_scale = s;
     }
     __syncthreads();
     const int s = s_scale;
@@ -726,7 +756,17 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     double2* Bm = reinterpret_cast<double2*>(R0);       // R0, R1 are dead
     for (int e = threadIdx.x; e < nn; e += blockDim.x) Bm[e] = make_double2(0.0, X1r[nn + e]); // Num = i Ur
     __syncthreads();
-    // Gauss-Jordan with partial pivoting (|re|+|im|, first maximum)
+    // Gauss-Jordan with partial pivoting (|re|+|im|, first maximum).  Each
+    // thread owns the same elements of [X0 | Bm] at every pivot: their (row,
+    // column) pairs are computed once, outside the pivot loop.
+    constexpr int kPer = (2 * kNMax * kNMax + kThreads - 1) / kThreads;
+    int er[kPer], ec[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int e = threadIdx.x + r * kThreads;
+        er[r] = e < n * 2 * n ? e / (2 * n) : -1;
+        ec[r] = e < n * 2 * n ? e % (2 * n) : 0;
+    }
     for (int k = 0; k < n; ++k) {
         if (threadIdx.x < 64) {
             const int i = k + threadIdx.x;
@@ -764,9 +804,10 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
         }
         for (int i = threadIdx.x; i < n; i += blockDim.x) s_f[i] = X0[i * n + k];
         __syncthreads();
-        for (int e = threadIdx.x; e < n * 2 * n; e += blockDim.x) {
-            const int i = e / (2 * n), jj = e % (2 * n);
-            if (i == k) continue;
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int i = er[r], jj = ec[r];
+            if (i < 0 || i == k) continue;
             const double2 f = s_f[i];
             if (jj < n) {
                 if (jj < k) continue;
