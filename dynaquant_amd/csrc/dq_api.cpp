@@ -173,9 +173,19 @@ struct dq_system {
 
 extern "C" const char* dq_last_error(void) { return g_err.c_str(); }
 
+// the matrix-core SpMM reads x rows and dense block values through buffer
+// resources with 32-bit byte offsets: one Krylov vector (n x padded batch,
+// 8 bytes per entry) and the union slots (64 bytes each) must stay below 2^31
+static bool mfma_offsets_fit(int64_t n, int64_t Bp, size_t union_slots)
+{
+    const uint64_t lim = (uint64_t)1 << 31;
+    return (uint64_t)n * (uint64_t)Bp * 8 < lim && (uint64_t)union_slots * 64 < lim;
+}
+
 // R = 16 (the matrix-core SpMM, four union columns per step, four steps in
 // flight): every block's union is padded to a multiple of 16 with zero-valued
 // copies of its first column.
+
 static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_t>& rowptr_v,
                                  const std::vector<int32_t>& col_v, const std::vector<double>& val_v, RowBlocks* out)
 {
@@ -206,6 +216,10 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
         }
         rbp[k + 1] = (int32_t)rbc.size();
     }
+    if (R == 16 && !mfma_offsets_fit(n, sys->d.Bp, rbc.size()))
+        return fail(DQ_ERR_INVALID_ARG, "the matrix-core SpMM (spmm_rows_per_wave = 16) addresses a Krylov vector "
+                                        "and the block values with 32-bit offsets: n * padded batch * 8 and "
+                                        "64 * union slots must stay below 2^31 bytes");
     std::vector<float> rbv32(rbv64.size());
     for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
     HIP_TRY(hipStreamSynchronize(sys->stream));
@@ -1484,6 +1498,13 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         return false;
     }
     if (t.spmm_rows_per_wave == 16) {
+        // 32-bit offsets (bounded above by nnz + 15 padding slots per 16-row block)
+        const size_t nblk16 = (size_t)(sys->n + 15) / 16;
+        const size_t nnz_max = std::max(sys->col.size(), sys->tk.p > 0 ? sys->tk.col_h.size() : (size_t)0);
+        if (!mfma_offsets_fit(sys->n, d.Bp, nnz_max + 15 * nblk16)) {
+            *why = "spmm_rows_per_wave = 16 needs n * padded batch * 8 and 64 * (nnz + 15 n / 16) below 2^31 bytes";
+            return false;
+        }
         // matrix-core SpMM: 1, 2 or 4 trajectories per lane = 16, 32, 64 per wave
         if (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2 && t.spmm_traj_per_lane != 4) {
             *why = "with spmm_rows_per_wave = 16, spmm_traj_per_lane must be 1, 2 or 4";

@@ -305,8 +305,9 @@ static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, i
 // sequence of MFMA accumulations whatever NG, so the slice width does not
 // change a bit of the result.  Against the 4-row VALU kernel: half the
 // gathered bytes per row at 2.7x the multiply-adds, which run on the matrix
-// cores.  On the 12,000-site dipole tube (B = 256) it runs level with that
-// kernel (42 us); it is an autotuner choice for denser couplings.
+// cores.  On the 12,000-site dipole tube (B = 256) it takes 39 us against
+// 43 us for that kernel (event-timed, whole batch), and inside the captured
+// step it leaves the vector ALU to the other slices' updates.
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
 
@@ -315,12 +316,34 @@ template <> struct XV<2> { using T = float2; };
 template <> struct XV<4> { using T = float4; };
 template <> struct XV<8> { struct T { float4 a, b; }; };
 
-template <int NG>
-__device__ __forceinline__ typename XV<NG>::T load_xv(const float* p)
+// x rows through a raw buffer resource (base = the wave's trajectory slice of
+// the vector slot, byte offsets below 2^31, reads past num_records return 0).
+// Being intrinsics, not plain loads, they stay where they are issued: a plain
+// "load here, use next iteration" is turned by the optimiser into a load at
+// the top of the next iteration, right before its use.
+__device__ __forceinline__ float4 buf_f4(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    if constexpr (NG == 2) return *reinterpret_cast<const float2*>(p);
-    else if constexpr (NG == 4) return *reinterpret_cast<const float4*>(p);
-    else return typename XV<8>::T{*reinterpret_cast<const float4*>(p), *reinterpret_cast<const float4*>(p + 4)};
+    const auto u = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]), __uint_as_float(u[2]), __uint_as_float(u[3]));
+}
+
+// A[i][k] of the step starting at union slot p (uniform byte offset 64 p)
+__device__ __forceinline__ float a_at(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int p)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, lane_off, 64 * p, 0));
+}
+
+template <int NG>
+__device__ __forceinline__ typename XV<NG>::T load_xv(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (NG == 2) {
+        const auto u = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return make_float2(__uint_as_float(u[0]), __uint_as_float(u[1]));
+    } else if constexpr (NG == 4) {
+        return buf_f4(r, off);
+    } else {
+        return typename XV<8>::T{buf_f4(r, off), buf_f4(r, off + 16)};
+    }
 }
 
 template <int NG>
@@ -354,10 +377,18 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
     const int Bp = d.Bp;
     const float2* __restrict__ x = reinterpret_cast<const float2*>(d.V) + (size_t)j * d.vs;
     float2* __restrict__ y = reinterpret_cast<float2*>(d.V) + (size_t)(j + 1) * d.vs;
-    const float* __restrict__ xf = reinterpret_cast<const float*>(x) + (size_t)ttile * 2 * TW + NG * nc;
-    const float* __restrict__ rv = d.rb_val32;
+    // this wave's trajectory slice of x: row r at byte r * 8 Bp, lane at 4 NG nc
+    const float* xs0 = reinterpret_cast<const float*>(x) + (size_t)ttile * 2 * TW;
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)xs0, (short)0, (int)((size_t)d.n * 8 * Bp - (size_t)ttile * 8 * TW), 0x00020000);
+    const uint32_t xlane = 4u * NG * nc;
+    // dense block values, 16 x 4 floats per step: lane i + 16 k reads A[i][k]
+    // at byte 64 p + 4 lane (p = the step's first union slot)
+    const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc((void*)d.rb_val32, (short)0, 0x7fffffff,
+                                                                        0x00020000);
+    const uint32_t alane = 4u * lane;
     const int32_t* __restrict__ rc = d.rb_col;
-    const size_t rowf = (size_t)2 * Bp;              // floats per row of x
+    const uint32_t rowb = 8u * Bp;                   // bytes per row of x
     const int nblk = (d.n + 15) / 16;
     const int rb = rtile * 4 + wave;
     const int b0 = ttile * TW + TL * nc;             // first of this lane's trajectories
@@ -370,39 +401,65 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
         mfma_f4 c[NG];
 #pragma unroll
         for (int g = 0; g < NG; ++g) c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
-        // column offsets come through the scalar cache (four per step, wave-
-        // uniform): a vector load there would make every gather wait for all
-        // the loads in flight before it.  The union is padded to a multiple of
-        // 4 kDepth columns, so every refill is issued (past the end it reloads
-        // a valid step) and the waits on the loads in flight are static.
-        auto cols4 = [&](int p) {
-            const int4 q4 = *reinterpret_cast<const int4*>(rc + p);
-            return make_int4(uniform(q4.x), uniform(q4.y), uniform(q4.z), uniform(q4.w));
+        // column offsets come through the scalar cache, wave-uniform, 4 kDepth
+        // (one 64-byte line) per load, issued one loop iteration before they
+        // are used: a vector load there would make every gather wait for all
+        // the loads in flight before it, and a scalar load used in the step
+        // that issues it stalls the wave on every line miss.  The union is
+        // padded to a multiple of 4 kDepth columns (and starts on one), so
+        // every refill is issued (past the end it reloads a valid group) and
+        // the waits on the loads in flight are static.
+        static_assert(kDepth == 4, "one int4 of column offsets per step of the group");
+        struct Cols { int4 s[kDepth]; };
+        auto cols = [&](int p) {
+            const int4* q4 = reinterpret_cast<const int4*>(rc + p);
+            Cols r;
+#pragma unroll
+            for (int q = 0; q < kDepth; ++q)
+                r.s[q] = make_int4(uniform(q4[q].x), uniform(q4[q].y), uniform(q4[q].z), uniform(q4[q].w));
+            return r;
         };
-        auto pick = [&](const int4& q4) { return kq == 0 ? q4.x : (kq == 1 ? q4.y : (kq == 2 ? q4.z : q4.w)); };
+        // byte offsets in 32-bit arithmetic (24-bit multiplies: columns and
+        // row strides are far below 2^24)
+        auto x_off = [&](int col) { return (uint32_t)__umul24((unsigned)col, rowb) + xlane; };
+        // lane group kq takes column kq of a step: selects, no branches
+        const bool k_odd = (kq & 1) != 0, k_high = (kq & 2) != 0;
+        auto pick = [&](const int4& q4) {
+            const int lo = k_odd ? q4.y : q4.x;
+            const int hi = k_odd ? q4.w : q4.z;
+            return k_high ? hi : lo;
+        };
         if (u1 > u0) {
             float av[kDepth];
             XT xv[kDepth];
+            {
+                const Cols c0 = cols(u0);
 #pragma unroll
-            for (int q = 0; q < kDepth; ++q) {
-                const int p = u0 + 4 * q;
-                av[q] = rv[(size_t)(p + kq) * 16 + nc];
-                xv[q] = load_xv<NG>(xf + (size_t)pick(cols4(p)) * rowf);
+                for (int q = 0; q < kDepth; ++q) {
+                    av[q] = a_at(ar, alane, u0 + 4 * q);
+                    xv[q] = load_xv<NG>(xr, x_off(pick(c0.s[q])));
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
+            // refill group of this iteration (columns pnext .. pnext + 4 kDepth - 1)
             int pnext = u0 + 4 * kDepth < u1 ? u0 + 4 * kDepth : u0;
-            int4 cn = cols4(pnext);
+            Cols cur = cols(pnext);
             for (int base = u0; base < u1; base += 4 * kDepth) {
+                const int pn2 = base + 8 * kDepth < u1 ? base + 8 * kDepth : u0;
+                const Cols nxt = cols(pn2);
 #pragma unroll
                 for (int q = 0; q < kDepth; ++q) {
 #pragma unroll
                     for (int g = 0; g < NG; ++g)
                         c[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], xv_at<NG>(xv[q], g), c[g], 0, 0, 0);
-                    av[q] = rv[(size_t)(pnext + kq) * 16 + nc];
-                    xv[q] = load_xv<NG>(xf + (size_t)pick(cn) * rowf);
-                    const int pn = base + 4 * q + 4 * kDepth + 4;
-                    pnext = pn < u1 ? pn : u0;
-                    cn = cols4(pnext);
+                    av[q] = a_at(ar, alane, pnext + 4 * q);
+                    xv[q] = load_xv<NG>(xr, x_off(pick(cur.s[q])));
+                    // step order kept: the refill of step q issues before the
+                    // multiplies of step q + 1, so kDepth steps stay in flight
+                    __builtin_amdgcn_sched_barrier(0);
                 }
+                pnext = pn2;
+                cur = nxt;
             }
         }
         const float g = (float)d.g;
