@@ -765,13 +765,26 @@ hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
     return hipGetLastError();
 }
 
+// resident waves per SIMD asked of the register allocator for the FP32
+// instantiations (each fits without spilling; one more wave than the
+// allocator's own choice for KW = 3, 4, 6)
+template <typename VT, int KW>
+constexpr int kUpdateWaves = sizeof(VT) != 4 ? 1 : (KW <= 2 ? 8 : KW == 3 ? 6 : KW == 4 ? 5 : KW <= 6 ? 4 : 1);
+
 // ---------------------------------------------------------------------------
 // Update of iteration j: v_{j+1} = s_j w - sum_{k<=j} (h_k s_k) v_k and the
 // FP64 partial Gram row <v_k|v_{j+1}>, |v_{j+1}|^2.  Block = rt_e rows x 16
-// trajectories; lane = 16 kg + t handles vectors k = kg, kg+4, ...
+// trajectories; lane = 16 kg + t handles vectors k = kg, kg+4, ... (KW =
+// ceil((j+1)/4) of them, so only the last group can hold a k > j).  A wave
+// takes rows r0 + wave, + 4, + 8, ...: two row buffers, each refilled 8 rows
+// ahead right after it is consumed, so the loads of two rows are in flight
+// while one is combined (a single buffer copied forward each row would make
+// every row wait for the loads just issued).  The coefficients stay in LDS,
+// which keeps the registers, and so the resident waves, for loads in flight.
 // ---------------------------------------------------------------------------
 template <typename VT, int KW>
-__global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<VT, KW>, 8)))
+void k_update_b(Dev d, int j)
 {
     using V2 = typename BVec2<VT>::T;
     __shared__ double2 s_c[4 * KW][16];
@@ -785,23 +798,27 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     const int t = lane & 15, kg = lane >> 4;
     const int b = ttile * 16 + t;
     const int Bp = d.Bp;
-    const size_t nB = (size_t)d.n * Bp;
     const V2* __restrict__ V = reinterpret_cast<const V2*>(d.V);
     V2* __restrict__ out = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * d.vs;
     // rows at or beyond the first (largest) n_active of the 16 sorted slots
     // are inactive for all of them: skipped, they stay 0
     const int r0 = rtile * d.rt_e, r1 = min(min(d.n, r0 + d.rt_e), (int)uniform(d.nact[ttile * 16]));
-    // software pipeline: the vectors of row i+4 load while row i is combined;
-    // the first row's loads are issued before the coefficients are staged
-    V2 nk[KW], nw;
-    if (r0 + wave < r1) {
-        const size_t e = (size_t)(r0 + wave) * Bp + b;
+    // every load is issued unconditionally (row index clamped to the wave's
+    // last row, vector index to j, w read by all four lane groups), so the
+    // wait before a row is "all but the other buffer's loads", never a drain
+    const int i0 = r0 + wave;
+    const int nrw = r1 > i0 ? (r1 - i0 + 3) / 4 : 0;       // rows of this wave
+    auto load_row = [&](int n, V2 (&v)[KW], V2& w) {
+        const size_t e = (size_t)(i0 + 4 * min(n, nrw - 1)) * Bp + b;
 #pragma unroll
-        for (int u = 0; u < KW; ++u) {
-            const int k = kg + 4 * u;
-            if (k <= j) nk[u] = V[(size_t)k * d.vs + e];
-        }
-        if (kg == 0) nw = out[e];
+        for (int u = 0; u < KW; ++u) v[u] = V[(size_t)min(kg + 4 * u, j) * d.vs + e];
+        w = out[e];
+    };
+    // the first two rows' loads are issued before the coefficients are staged
+    V2 va[KW], vb[KW], wa, wb;
+    if (nrw > 0) {
+        load_row(0, va, wa);
+        load_row(1, vb, wb);
     }
     for (int q = threadIdx.x; q < 16 * (j + 1); q += kThreads) {
         const int k = q >> 4, tt = q & 15, bb = ttile * 16 + tt;
@@ -812,30 +829,11 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     }
     __syncthreads();
     const double sw = s_w[t];
-    double2 cf[KW];
-#pragma unroll
-    for (int u = 0; u < KW; ++u) {
-        const int k = kg + 4 * u;
-        cf[u] = (k <= j) ? s_c[k][t] : make_double2(0.0, 0.0);
-    }
     double acc[2 * KW + 1];
 #pragma unroll
     for (int q = 0; q < 2 * KW + 1; ++q) acc[q] = 0.0;
-    for (int i = r0 + wave; i < r1; i += 4) {
-        const size_t e = (size_t)i * Bp + b;
-        V2 vk[KW];
-#pragma unroll
-        for (int u = 0; u < KW; ++u) vk[u] = nk[u];
-        const V2 w = nw;
-        if (i + 4 < r1) {
-            const size_t en = e + (size_t)4 * Bp;
-#pragma unroll
-            for (int u = 0; u < KW; ++u) {
-                const int k = kg + 4 * u;
-                if (k <= j) nk[u] = V[(size_t)k * d.vs + en];
-            }
-            if (kg == 0) nw = out[en];
-        }
+    auto do_row = [&](int n, const V2 (&vk)[KW], const V2& w) {
+        const size_t e = (size_t)(i0 + 4 * n) * Bp + b;
         double pr = 0.0, pi = 0.0;
         if (kg == 0) {
             pr = sw * (double)w.x;
@@ -844,10 +842,11 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k <= j) {
+            if (u < KW - 1 || k <= j) {
+                const double2 cf = s_c[k][t];
                 const double xr = (double)vk[u].x, xi = (double)vk[u].y;
-                pr += cf[u].x * xr - cf[u].y * xi;
-                pi += cf[u].x * xi + cf[u].y * xr;
+                pr += cf.x * xr - cf.y * xi;
+                pi += cf.x * xi + cf.y * xr;
             }
         }
         pr += __shfl_xor(pr, 16, 64);
@@ -862,13 +861,19 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k <= j) {
+            if (u < KW - 1 || k <= j) {
                 const double xr = (double)vk[u].x, xi = (double)vk[u].y;
                 acc[2 * u] += xr * nr + xi * ni;
                 acc[2 * u + 1] += xr * ni - xi * nr;
             }
         }
         if (kg == 0) acc[2 * KW] += nr * nr + ni * ni;
+    };
+    for (int n = 0; n < nrw; n += 2) {
+        do_row(n, va, wa);
+        load_row(n + 2, va, wa);
+        if (n + 1 < nrw) do_row(n + 1, vb, wb);
+        load_row(n + 3, vb, wb);
     }
     // combine the four waves (rows) per lane, then write q = 2k, 2k+1 and nu
 #pragma unroll
@@ -894,21 +899,26 @@ __global__ __launch_bounds__(kThreads) void k_update_b(Dev d, int j)
     }
 }
 
+template <typename VT>
+static void launch_update_bt(const Dev& d, int j, hipStream_t s)
+{
+    const dim3 grid(bpad8(d.ntile_e * (d.sn / 16))), block(kThreads);
+    switch ((j + 4) / 4) {          // KW = ceil((j + 1) / 4)
+    case 1: hipLaunchKernelGGL((k_update_b<VT, 1>), grid, block, 0, s, d, j); break;
+    case 2: hipLaunchKernelGGL((k_update_b<VT, 2>), grid, block, 0, s, d, j); break;
+    case 3: hipLaunchKernelGGL((k_update_b<VT, 3>), grid, block, 0, s, d, j); break;
+    case 4: hipLaunchKernelGGL((k_update_b<VT, 4>), grid, block, 0, s, d, j); break;
+    case 5: hipLaunchKernelGGL((k_update_b<VT, 5>), grid, block, 0, s, d, j); break;
+    case 6: hipLaunchKernelGGL((k_update_b<VT, 6>), grid, block, 0, s, d, j); break;
+    case 7: hipLaunchKernelGGL((k_update_b<VT, 7>), grid, block, 0, s, d, j); break;
+    default: hipLaunchKernelGGL((k_update_b<VT, 8>), grid, block, 0, s, d, j); break;
+    }
+}
+
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s)
 {
-    dim3 grid(bpad8(d.ntile_e * (d.sn / 16))), block(kThreads);
-    const int nv = j + 1;
-    if (d.mixed) {
-        if (nv <= 8) hipLaunchKernelGGL((k_update_b<float, 2>), grid, block, 0, s, d, j);
-        else if (nv <= 16) hipLaunchKernelGGL((k_update_b<float, 4>), grid, block, 0, s, d, j);
-        else if (nv <= 24) hipLaunchKernelGGL((k_update_b<float, 6>), grid, block, 0, s, d, j);
-        else hipLaunchKernelGGL((k_update_b<float, 8>), grid, block, 0, s, d, j);
-    } else {
-        if (nv <= 8) hipLaunchKernelGGL((k_update_b<double, 2>), grid, block, 0, s, d, j);
-        else if (nv <= 16) hipLaunchKernelGGL((k_update_b<double, 4>), grid, block, 0, s, d, j);
-        else if (nv <= 24) hipLaunchKernelGGL((k_update_b<double, 6>), grid, block, 0, s, d, j);
-        else hipLaunchKernelGGL((k_update_b<double, 8>), grid, block, 0, s, d, j);
-    }
+    if (d.mixed) launch_update_bt<float>(d, j, s);
+    else launch_update_bt<double>(d, j, s);
     return hipGetLastError();
 }
 
