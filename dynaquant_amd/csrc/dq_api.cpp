@@ -357,6 +357,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     const size_t ts_max = plain ? (size_t)d.ntile_s : (size_t)((d.n + 4 * dq::kRBMin - 1) / (4 * dq::kRBMin));
     size_t part = std::max(nq * te_max, (size_t)2 * ts_max) * d.Bp;
     ALLOC(&d.part, part);
+    ALLOC(&d.part_s, (size_t)2 * ts_max * d.Bp);
     ALLOC(&d.red, (nq + 2) * d.Bp);
     ALLOC(&d.step, 1);
     // trajectories: ids 0..B-1, all sites active; padding trajectories inactive
@@ -880,11 +881,16 @@ extern "C" {
 static hipError_t enqueue_lanczos_pade(dq_system* sys, const Dev& d, hipStream_t st)
 {
     hipError_t e;
+    // batched: the SpMM partials of iteration j and the Gram partials of
+    // update j-1 are reduced in one launch before the scalars (the last
+    // update's |v_m|^2 is summed by the Pade kernel); plain: the scalars
+    // kernel sums the SpMM tiles and each update is followed by its reduction
     for (int j = 0; j < d.m; ++j) {
         if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_spmm(d, j, 0, st); })) != hipSuccess) return e;
+        if (!d.plain && (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce_all(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_SCALARS, [&] { return dq::launch_scalars_h(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_UPDATE, [&] { return dq::launch_update(d, j, st); })) != hipSuccess) return e;
-        if ((e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
+        if (d.plain && (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
     }
     if ((e = timed(sys, DQ_K_SPMM_LAST, [&] { return dq::launch_spmm(d, d.m, 1, st); })) != hipSuccess) return e;
     return timed(sys, DQ_K_PADE, [&] { return dq::launch_pade(d, st); });
@@ -1587,9 +1593,10 @@ static hipError_t enqueue_tuning_cycle(dq_system* sys, const Dev& d, hipStream_t
     hipError_t e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
+        if (!d.plain && (e = dq::launch_reduce_all(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
-        if ((e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;
+        if (d.plain && (e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;
     }
     if ((e = dq::launch_spmm(d, d.m, 1, st)) != hipSuccess) return e;
     return dq::launch_combine(d, st);

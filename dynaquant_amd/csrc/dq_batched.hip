@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
 #pragma unroll
             for (int t = 0; t < TPL; ++t) {
                 const double v = (lds[0][q][t][lane] + lds[1][q][t][lane]) + (lds[2][q][t][lane] + lds[3][q][t][lane]);
-                d.part[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
+                d.part_s[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
             }
         }
     }
@@ -520,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
 #pragma unroll
             for (int t = 0; t < TL; ++t) {
                 const double v = (lds[0][q][t][nc] + lds[1][q][t][nc]) + (lds[2][q][t][nc] + lds[3][q][t][nc]);
-                d.part[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
+                d.part_s[((size_t)q * d.ntile_s + rtile) * Bp + b0 + t] = v;
             }
         }
     }
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = d.s0 + blockIdx.x * 64 + lane;
     const int Bp = d.Bp;
-    // <v_j|w> reduced over the SpMM tiles by k_reduce_wide into red[kRedC..]
+    // <v_j|w> reduced over the SpMM tiles by k_reduce_all into red[kRedC..]
     const double cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
     const double ci = d.red[((size_t)kRedC(d.m) + 1) * Bp + b];
     const bool writer = blockIdx.y == 0;
@@ -721,18 +721,14 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
     }
 }
 
-// red[dst + q][b] = sum over tiles of part[q][tile][b]: 16 waves per block,
-// four load chains per lane, fixed combination order
-__global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
+// out[b] = sum over tiles t < ntile of p[t Bp + b] (p = one quantity's tile
+// column): 16 waves per block, eight independent loads in flight per lane,
+// fixed summation order; block = 64 trajectories starting at b - lane
+__device__ __forceinline__ void wide_tile_sum(const double* p, int ntile, int Bp, double* out)
 {
     __shared__ double lds[16][64];
-    const int q = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = d.s0 + blockIdx.x * 64 + lane;
-    const int Bp = d.Bp;
-    // eight independent loads in flight per lane, fixed summation order
     double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double* p = d.part + (size_t)q * ntile * Bp + b;
     int t = wave;
     for (; t + 112 < ntile; t += 128) {
 #pragma unroll
@@ -746,8 +742,38 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
     if (wave == 0) {
         double s = 0.0;
         for (int w = 0; w < 16; ++w) s += lds[w][lane];
-        d.red[((size_t)dst + q) * Bp + b] = s;
+        *out = s;
     }
+}
+
+// red[dst + q][b] = sum over tiles of part[q][tile][b]
+__global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
+{
+    const int q = blockIdx.y;
+    const int b = d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Bp = d.Bp;
+    wide_tile_sum(d.part + (size_t)q * ntile * Bp + b, ntile, Bp, d.red + ((size_t)dst + q) * Bp + b);
+}
+
+// blockIdx.y < 2: <v_j|w> (re, im) from the SpMM tiles; else quantity
+// y - 2 of update j-1 (Gram row re/im pairs, then |v_j|^2) from its tiles
+__global__ __launch_bounds__(1024) void k_reduce_all(Dev d, int j)
+{
+    const int y = blockIdx.y;
+    const int b = d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int Bp = d.Bp;
+    if (y < 2)
+        wide_tile_sum(d.part_s + (size_t)y * d.ntile_s * Bp + b, d.ntile_s, Bp,
+                      d.red + ((size_t)kRedC(d.m) + y) * Bp + b);
+    else
+        wide_tile_sum(d.part + (size_t)(y - 2) * d.ntile_e * Bp + b, d.ntile_e, Bp, d.red + (size_t)(y - 2) * Bp + b);
+}
+
+hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s)
+{
+    const int nq = 2 + (j >= 1 ? 2 * j + 1 : 0);
+    hipLaunchKernelGGL(k_reduce_all, dim3(d.sn / 64, nq), dim3(1024), 0, s, d, j);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s)
@@ -756,10 +782,9 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
     return hipGetLastError();
 }
 
+// after launch_reduce_all(d, j)
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 {
-    hipError_t e = launch_reduce_wide(d, 2, d.ntile_s, kRedC(d.m), s);
-    if (e != hipSuccess) return e;
     const int ny = j > 0 ? (j + 3) / 4 : 1;
     hipLaunchKernelGGL(k_scalars_hb, dim3(d.sn / 64, ny), dim3(kThreads), 0, s, d, j);
     return hipGetLastError();

@@ -80,6 +80,7 @@ struct Dev {
     int* broke;                     // [Bp]
     int* status;                    // [1] sticky error flags
     double* part;                   // partial sums
+    double* part_s;                 // batched SpMM partials <v_j|w> [q < 2][tile][Bp]
     double* red;                    // [q][Bp] reduced sums
     // observables
     const int32_t* group;           // [n]
@@ -123,6 +124,10 @@ hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s);
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s);
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s);
 hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s);
+// Lanczos iteration j, batched: <v_j|w> over the SpMM tiles (part_s) into
+// red[kRedC(m)..+1] and, for j >= 1, the Gram row and |v_j|^2 of update j-1
+// (part, 2j+1 quantities) into red[0..2j], in one launch
+hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
 hipError_t launch_combine_b(const Dev& d, hipStream_t s);
 hipError_t launch_pade(const Dev& d, hipStream_t s);

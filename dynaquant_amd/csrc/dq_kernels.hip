@@ -851,20 +851,27 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
     __shared__ double s_hq;
     const int b = d.s0 + blockIdx.x;
     const int m = d.m, Bp = d.Bp;
-    // |H v_m|^2 summed over the last SpMM's tiles
-    double hq2 = 0.0;
+    // |H v_m|^2 summed over the last SpMM's tiles and, batched, |v_m|^2 over
+    // the last update's tiles (the plain path has it reduced in red[2m])
+    double hq2 = 0.0, nu_b = 0.0;
     {
-        double s = 0.0;
-        for (int t = threadIdx.x; t < d.ntile_s; t += kThreads) s += d.part[(size_t)t * Bp + b];
+        const double* ps = d.plain ? d.part : d.part_s;
+        double s = 0.0, v = 0.0;
+        for (int t = threadIdx.x; t < d.ntile_s; t += kThreads) s += ps[(size_t)t * Bp + b];
+        if (!d.plain)
+            for (int t = threadIdx.x; t < d.ntile_e; t += kThreads)
+                v += d.part[((size_t)(2 * m) * d.ntile_e + t) * Bp + b];
         s = wave_sum(s);
-        __shared__ double l4[4];
-        if ((threadIdx.x & 63) == 0) l4[threadIdx.x >> 6] = s;
+        v = wave_sum(v);
+        __shared__ double l4[2][4];
+        if ((threadIdx.x & 63) == 0) { l4[0][threadIdx.x >> 6] = s; l4[1][threadIdx.x >> 6] = v; }
         __syncthreads();
-        hq2 = (l4[0] + l4[1]) + (l4[2] + l4[3]);
+        hq2 = (l4[0][0] + l4[0][1]) + (l4[0][2] + l4[0][3]);
+        nu_b = (l4[1][0] + l4[1][1]) + (l4[1][2] + l4[1][3]);
     }
     if (threadIdx.x == 0) {
-        // beta_{m-1} and s_m from the last update's reduced Gram sums
-        const double nu = d.red[(size_t)(2 * m) * Bp + b];
+        // beta_{m-1} and s_m from the last update's Gram sums
+        const double nu = d.plain ? d.red[(size_t)(2 * m) * Bp + b] : nu_b;
         double bet = __builtin_sqrt(nu);
         const double bprev = m >= 2 ? d.beta[(size_t)(m - 2) * Bp + b] : 0.0;
         const double aprev = d.alpha[(size_t)(m - 1) * Bp + b];
