@@ -689,12 +689,26 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
                 d.beta[(size_t)(j - 1) * Bp + b] = bet;
                 d.sc[(size_t)j * Bp + b] = sj;
             }
-            for (int l = 0; l < j; ++l) {
-                const double f = d.sc[(size_t)l * Bp + b] * sj;
-                const double2 G = make_double2(f * d.red[(size_t)(2 * l) * Bp + b],
-                                               f * d.red[(size_t)(2 * l + 1) * Bp + b]);
-                s_G[l][lane] = G;
-                if (writer) d.gram[(size_t)l * Bp + b] = G;
+            // eight l per round: the loads of a round are issued together
+            for (int l0 = 0; l0 < j; l0 += 8) {
+                double sl[8], gr[8], gi[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int l = min(l0 + u, j - 1);
+                    sl[u] = d.sc[(size_t)l * Bp + b];
+                    gr[u] = d.red[(size_t)(2 * l) * Bp + b];
+                    gi[u] = d.red[(size_t)(2 * l + 1) * Bp + b];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int l = l0 + u;
+                    if (l < j) {
+                        const double f = sl[u] * sj;
+                        const double2 G = make_double2(f * gr[u], f * gi[u]);
+                        s_G[l][lane] = G;
+                        if (writer) d.gram[(size_t)l * Bp + b] = G;
+                    }
+                }
             }
             s_G[j][lane] = make_double2(1.0, 0.0);
             s_sj[lane] = bet;          // beta_{j-1}, needed by k = j-1 below
@@ -711,11 +725,20 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
         const double bk = (k == j - 1) ? s_sj[lane] : d.beta[(size_t)k * Bp + b];
         const double2 Gk1 = s_G[k + 1][lane];
         double sr = bk * Gk1.x, si = bk * Gk1.y;
-        for (int l = 0; l <= k; ++l) {
-            const double2 a = d.hco[((size_t)k * kMaxM + l) * Bp + b];
-            const double2 G = s_G[l][lane];
-            sr += a.x * G.x + a.y * G.y;
-            si += a.x * G.y - a.y * G.x;
+        // h^(k)_l in rounds of eight independent loads, summed in order of l
+        for (int l0 = 0; l0 <= k; l0 += 8) {
+            double2 a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = d.hco[((size_t)k * kMaxM + min(l0 + u, k)) * Bp + b];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int l = l0 + u;
+                if (l <= k) {
+                    const double2 G = s_G[l][lane];
+                    sr += a[u].x * G.x + a[u].y * G.y;
+                    si += a[u].x * G.y - a[u].y * G.x;
+                }
+            }
         }
         d.hco[((size_t)j * kMaxM + k) * Bp + b] = make_double2(sr, si);
     }
