@@ -246,7 +246,7 @@ def main():
     upd_name, spmm_name = update_kernel_name(tuning), spmm_kernel_name(tuning)
     upd_roof = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(upd_name),
-                "kernel": upd_name + " (KW = 2/4/6 for j+1 <= 8/16/24)",
+                "kernel": upd_name + " (KW = ceil((j+1)/4) = 1..6 for j = 0..23)",
                 "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms}
     spmm_roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(spmm_name),
