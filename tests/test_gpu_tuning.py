@@ -171,3 +171,31 @@ def test_matrix_core_spmm_ragged_and_sliced(engine):
     st = out["mfma"][1]
     for b in range(128):
         assert np.all(st[b, na[b]:] == 0)
+
+
+def test_matrix_core_spmm_refused_beyond_32bit_offsets(engine):
+    """the matrix-core SpMM addresses one Krylov vector through a buffer
+    resource with 32-bit byte offsets, so a system with n * padded batch * 8 >=
+    2^31 (n = 2^20 + 64 sites of an open chain, B = 256: about 22 GB of device
+    memory) refuses that choice with an error naming the limit, while the VALU
+    SpMM choices stay available"""
+    n, B = (1 << 20) + 64, 256
+    assert n * B * 8 >= 1 << 31
+    i = np.arange(n)
+    nb = [np.stack([i[1:], i[:-1]], 1), np.stack([i[:-1], i[1:]], 1)]
+    pairs = np.concatenate(nb)
+    pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
+    rp = np.zeros(n + 1, np.int32)
+    np.add.at(rp, pairs[:, 0] + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    col = pairs[:, 1].astype(np.int32)
+    val = np.full(len(col), -100.0)
+    s = engine.System(n, 4, 1.0, n_traj=B, precision=engine.MIXED, seed=1)
+    try:
+        s.set_hamiltonian(rp, col, val, np.zeros(n))
+        with pytest.raises(engine.DQError, match="2\\^31"):
+            s.set_tuning(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_phases=1, spmm_pipeline=0)
+        s.set_tuning(spmm_rows_per_wave=8, spmm_traj_per_lane=2, spmm_phases=1, spmm_pipeline=0)
+        assert s.tuning()["spmm_rows_per_wave"] == 8
+    finally:
+        s.close()
