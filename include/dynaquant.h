@@ -282,7 +282,10 @@ dq_status dq_get_kernel_times(dq_system *sys, double *ms, int64_t *launches);
  * single-trajectory path has fixed shapes and ignores them. */
 typedef struct dq_tuning {
     int32_t spmm_rows_per_wave;  /* 2, 4 or 8 rows whose column union one wavefront gathers;
-                                    16: the matrix-core SpMM (mixed precision only) */
+                                    16: the matrix-core SpMM (mixed precision only; its
+                                    32-bit offsets need n * padded batch * 8 bytes and
+                                    64 bytes per union slot below 2^31, else
+                                    DQ_ERR_INVALID_ARG) */
     int32_t spmm_traj_per_lane;  /* 1 or 2 (2: FP32 only, padded batch a multiple of 128) */
     int32_t spmm_phases;         /* trajectory slices of one SpMM run as 1, 2 or 4 launches */
     int32_t tile_rows;           /* 32, 64, 128 or 256 rows per workgroup of the update, combine
