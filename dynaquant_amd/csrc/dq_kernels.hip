@@ -260,10 +260,21 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
                 // Kubo-Anderson / OU, S Euler-Maruyama substeps (gpuSysHelper.cu:73-128)
                 const double invTau = d.inv_tau[i], coeff = d.coeff[i];
                 if (coeff != 0.0 || invTau != 0.0) {
+                    // substep s uses normal s & 3 of Philox block s >> 2; one
+                    // block per four substeps, and only the Box-Muller pair in
+                    // use is evaluated (the same values normal4 returns)
+                    uint64_t c[4] = {0, 0, 0, 0};
+                    double z[2] = {0.0, 0.0};
                     for (int s = 0; s < d.S; ++s) {
-                        double z[4];
-                        normal4(d.seed, tid_b, (uint64_t)i, (uint64_t)step, kStreamOu, (uint64_t)(s >> 2), z);
-                        double xi = z[s & 3];
+                        if ((s & 3) == 0) {
+                            c[0] = (uint64_t)i; c[1] = (uint64_t)step; c[2] = kStreamOu; c[3] = (uint64_t)(s >> 2);
+                            philox4x64_10(c, d.seed, tid_b);
+                        }
+                        if ((s & 1) == 0) {
+                            if ((s & 2) == 0) box_muller(c[0], c[1], &z[0], &z[1]);
+                            else box_muller(c[2], c[3], &z[0], &z[1]);
+                        }
+                        const double xi = z[s & 1];
                         cur = cur - cur * invTau * d.subdt + coeff * xi * d.sqrt_subdt;
                     }
                     d.de[e] = cur;
