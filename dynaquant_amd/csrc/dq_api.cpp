@@ -1585,12 +1585,20 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     return DQ_OK;
 }
 
-// one Lanczos cycle without the kernels that touch the propagated state
-// (noise, Pade error sum, renormalisation): SpMMs, scalars, updates, reductions
-// and the combine (whose psi output the caller restores)
+// one Lanczos cycle without the Pade error sum, the renormalisation and the
+// observables: once the noise is set up, the noise prologue (whose workgroups
+// follow tile_rows like the update's), then SpMMs, scalars, updates,
+// reductions and the combine.  The caller saves and restores psi, the noise
+// offsets and the diagonal around it.
+static bool tuning_with_noise(const dq_system* sys)
+{
+    return sys->noise_ready() && sys->have_disorder;
+}
+
 static hipError_t enqueue_tuning_cycle(dq_system* sys, const Dev& d, hipStream_t st)
 {
     hipError_t e;
+    if (tuning_with_noise(sys) && (e = dq::launch_prologue(d, 0, 1, 0, 1, st)) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
         if (!d.plain && (e = dq::launch_reduce_all(d, j, st)) != hipSuccess) return e;
@@ -1709,13 +1717,20 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     sys->drop_graph();
     HIP_TRY(hipStreamSynchronize(sys->stream));
     const size_t nB = (size_t)d.n * d.Bp;
-    double2* saved = nullptr;
-    HIP_TRY(hipMalloc(&saved, sizeof(double2) * nB));
+    // saved state: psi, then the noise offsets and the diagonal
+    const size_t diag_bytes = (d.mixed ? sizeof(float) : sizeof(double)) * nB;
+    char* saved_raw = nullptr;
+    HIP_TRY(hipMalloc(&saved_raw, sizeof(double2) * nB + sizeof(double) * nB + diag_bytes));
+    double2* saved = reinterpret_cast<double2*>(saved_raw);
+    double* saved_de = reinterpret_cast<double*>(saved_raw + sizeof(double2) * nB);
+    void* saved_diag = saved_raw + sizeof(double2) * nB + sizeof(double) * nB;
     dq_tuning orig = best;
     dq_status st = DQ_OK;
     double best_ms = 1e300;
     std::vector<std::pair<float, dq_tuning>> timed_choices;
     hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(saved_de, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(saved_diag, d.diag, diag_bytes, hipMemcpyDeviceToDevice, sys->stream);
     for (int R : {4, 2, 8, 16}) {
         for (int tpl : {2, 1, 4}) {
             for (int ph : {1, 2, 4}) {
@@ -1791,8 +1806,10 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
         }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(d.psi, saved, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d.de, saved_de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d.diag, saved_diag, diag_bytes, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
-    hipFree(saved);
+    hipFree(saved_raw);
     if (e != hipSuccess) return fail(DQ_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
     if (st != DQ_OK) { apply_tuning(sys, orig); return st; }
     DQ_TRY(apply_tuning(sys, best));
