@@ -302,8 +302,11 @@ dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);
 /* Validates every field; drops a captured graph. */
 dq_status dq_set_tuning(dq_system *sys, const dq_tuning *t);
 /* Times every valid dq_tuning on the system's current Hamiltonian and batch
- * (reps Lanczos cycles each, the state propagated by dq_run is left as it
- * was), applies the fastest and returns it in *chosen.  With cache_path, a
+ * (reps cycles each: the noise prologue once the noise is set up, the
+ * Lanczos kernels and the combine; psi, the noise offsets and the diagonal
+ * are saved and restored, so the state propagated by dq_run is left as it
+ * was), applies the fastest and returns it in *chosen.  Allocates a scratch
+ * copy of that state for the duration of the call.  With cache_path, a
  * line keyed by device, n, nnz, padded batch, m and precision is looked up
  * first and, after a search, appended.  Needs set_hamiltonian. */
 dq_status dq_autotune(dq_system *sys, int32_t reps, const char *cache_path, dq_tuning *chosen);
