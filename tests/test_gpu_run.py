@@ -169,3 +169,38 @@ def test_errors_are_reported(engine):
     with pytest.raises(engine.DQError) as ei:
         big.run(1)
     assert ei.value.code == 6
+
+
+def test_deepest_krylov_space_batched(engine, oracle):
+    """m = 32, the deepest Krylov space (34 x 34 augmented Pade matrix; the
+    batched update's largest vector-group counts, KW = 7 and 8): FP64 batched
+    against the oracle per trajectory, and the mixed path against FP64 with
+    the matrix-core SpMM and with the VALU SpMM"""
+    sysd = ring_tube(engine)
+    n, steps, B, m, dt = 600, 60, 4, 32, 2.0
+    psi0 = np.zeros(n, complex); psi0[[0, 37]] = np.sqrt([0.6, 0.4])
+    grp = (np.arange(n) // 150).astype(np.int32)
+    ids = [0, 7, 2, 11]
+    s = make_engine(engine, sysd, m, dt, B=B, force_batched=True, traj_ids=ids)
+    s.set_state(psi0)
+    s.set_observables(grp, 4, 5, steps)
+    s.run(steps, graph=True)
+    pops64, err64 = s.populations(), s.error_sum()
+    s.close()
+    p, sites = make_oracle(oracle, sysd, m, dt)
+    rb = oracle.run_batch(p, sites, sysd["rp"], sysd["col"], sysd["val"], psi0, ids, steps, 5, grp, 4)
+    err = np.abs(pops64 - rb["pops"]).max()
+    erel = np.abs(err64 - rb["err"]).max() / np.abs(rb["err"]).max()
+    print("m=32 batched FP64 vs oracle: max |P - P_oracle| %.3e, error-sum rel %.3e" % (err, erel))
+    assert err <= 1e-10 and erel <= 1e-6
+    for tun in (dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_phases=1, spmm_pipeline=0),
+                dict(spmm_rows_per_wave=4, spmm_traj_per_lane=1, spmm_phases=1, spmm_pipeline=0)):
+        s = make_engine(engine, sysd, m, dt, B=64, precision=engine.MIXED, traj_ids=ids + list(range(100, 160)))
+        s.set_tuning(**tun)
+        s.set_state(psi0)
+        s.set_observables(grp, 4, 5, steps)
+        s.run(steps, graph=True)
+        d = np.abs(s.populations()[:B] - pops64).max()
+        s.close()
+        print("m=32 mixed (rows per wave %d) vs FP64: %.3e" % (tun["spmm_rows_per_wave"], d))
+        assert d <= 1e-6
