@@ -827,8 +827,9 @@ constexpr int kUpdateWaves = sizeof(VT) != 4 ? 1 : (KW <= 2 ? 8 : KW == 3 ? 6 : 
 // takes rows r0 + wave, + 4, + 8, ...: two row buffers, each refilled 8 rows
 // ahead right after it is consumed, so the loads of two rows are in flight
 // while one is combined (a single buffer copied forward each row would make
-// every row wait for the loads just issued).  The coefficients stay in LDS,
-// which keeps the registers, and so the resident waves, for loads in flight.
+// every row wait for the loads just issued).  The coefficients -h_k s_k are
+// staged once per block in LDS (the compiler keeps each lane's KW of them in
+// registers across the row loop).
 // ---------------------------------------------------------------------------
 template <typename VT, int KW>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<VT, KW>, 8)))
