@@ -185,7 +185,6 @@ static bool mfma_offsets_fit(int64_t n, int64_t Bp, size_t union_slots)
 // R = 16 (the matrix-core SpMM, four union columns per step, four steps in
 // flight): every block's union is padded to a multiple of 16 with zero-valued
 // copies of its first column.
-
 static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_t>& rowptr_v,
                                  const std::vector<int32_t>& col_v, const std::vector<double>& val_v, RowBlocks* out)
 {
