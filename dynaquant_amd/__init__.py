@@ -32,7 +32,8 @@ EXPORTS = [
     "dq_read_hamiltonian_file", "dq_read_state_file", "dq_read_dipole_file", "dq_read_site_columns",
     "dq_read_spectral_density", "dq_parse_infile", "dq_write_pops_file", "dq_read_pops_file",
     "dq_read_position_file", "dq_cd_vectors", "dq_system_set_participation", "dq_get_participation",
-    "dq_system_set_taylor_krylov",
+    "dq_system_set_taylor_krylov", "dq_system_snapshot", "dq_system_restore", "dq_system_set_bras",
+    "dq_spectrum_weighted",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -59,12 +60,13 @@ class Tuning(C.Structure):
     _fields_ = [
         ("spmm_rows_per_wave", C.c_int32), ("spmm_traj_per_lane", C.c_int32),
         ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("spmm_pipeline", C.c_int32),
-        ("concurrent_slices", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("concurrent_slices", C.c_int32), ("spmm_window_rows", C.c_int32),
+        ("spmm_window_waves", C.c_int32),
     ]
 
 
 TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows", "spmm_pipeline",
-               "concurrent_slices"]
+               "concurrent_slices", "spmm_window_rows", "spmm_window_waves"]
 
 
 class TubeSpec(C.Structure):
@@ -234,6 +236,41 @@ class System:
         a = np.zeros(2 * self.n * self.B)
         _check(lib().dq_system_get_state(self._h, _p(a)))
         return a.view(np.complex128).reshape(self.B, self.n)
+
+    def snapshot(self):
+        """resume point: dict(psi [B, n], noise [B, n], err_sum [B], step)"""
+        psi = np.zeros(2 * self.B * self.n)
+        noise = np.zeros(self.B * self.n)
+        err = np.zeros(self.B)
+        step = C.c_int64()
+        _check(lib().dq_system_snapshot(self._h, _p(psi), _p(noise), _p(err), C.byref(step)))
+        return dict(psi=psi.view(np.complex128).reshape(self.B, self.n), noise=noise.reshape(self.B, self.n),
+                    err_sum=err, step=int(step.value))
+
+    def restore(self, snap):
+        """continue from a snapshot (after reset_disorder)"""
+        psi = c2i(np.asarray(snap["psi"]).reshape(-1))
+        noise = _c(np.asarray(snap["noise"]).reshape(-1), np.float64)
+        err = None if snap.get("err_sum") is None else _c(snap["err_sum"], np.float64)
+        _check(lib().dq_system_restore(self._h, _p(psi), _p(noise), _p(err), C.c_int64(int(snap["step"]))))
+
+    def set_bras(self, bras, bra_of_traj):
+        """C_b(n) = <bras[bra_of_traj[b]]|psi_b(n)> (after set_observables)"""
+        bras = np.atleast_2d(np.asarray(bras, np.complex128))
+        idx = _c(np.asarray(bra_of_traj, np.int32), np.int32)
+        if idx.size != self.B or bras.shape[1] != self.n:
+            raise ValueError("bra_of_traj needs B entries and bras n columns")
+        _check(lib().dq_system_set_bras(self._h, int(bras.shape[0]), _p(c2i(bras.ravel())), _p(idx)))
+
+    def spectrum_weighted(self, sigma_fs, weight, nfft):
+        """(omega [nfft], spectrum [nfft]) of sum_b weight[b] C_b(n), scaled 1/T"""
+        om = np.zeros(nfft)
+        sp = np.zeros(2 * nfft)
+        w = _c(weight, np.float64)
+        if w.size != self.B:
+            raise ValueError("one weight per trajectory")
+        _check(lib().dq_spectrum_weighted(self._h, C.c_double(sigma_fs), _p(w), int(nfft), _p(om), _p(sp)))
+        return om, sp.view(np.complex128)
 
     def set_observables(self, group=None, ngroups=0, every=0, max_steps=0, bra=None):
         g = None if group is None else _c(group, np.int32)
@@ -463,6 +500,21 @@ def spectrum_runs(pos, mu, kind="absorption"):
     rd, mt, rt = cd_vectors(pos, mu)
     return [((rd[:, d] / rt[d]).astype(np.complex128), (mu[:, d] / mt[d]).astype(np.complex128),
              CD_PREFACTOR * mt[d] * rt[d]) for d in range(3)]
+
+
+def direction_batch(pos, mu, per_direction, kind="absorption"):
+    """the three direction ensembles of an orientationally averaged spectrum
+    (spectraClass.cu:437-496, 699-744) as one batch of 3 * per_direction
+    trajectories, direction d taking trajectories d * per_direction ..:
+    returns (psi0 [3 B_d, n], bras [3, n], bra_of_traj [3 B_d], weight [3 B_d])
+    for set_state, set_bras and spectrum_weighted; weight = scale_d / (3 B_d),
+    so spectrum_weighted is the mean of the three direction spectra."""
+    runs = spectrum_runs(pos, mu, kind)
+    psi0 = np.repeat(np.stack([r[0] for r in runs]), per_direction, axis=0)
+    bras = np.stack([r[1] for r in runs])
+    idx = np.repeat(np.arange(3, dtype=np.int32), per_direction)
+    weight = np.repeat(np.array([r[2] for r in runs]) / (3.0 * per_direction), per_direction)
+    return psi0, bras, idx, weight
 
 
 def read_site_columns(path, n, ncol):

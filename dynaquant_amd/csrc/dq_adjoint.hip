@@ -686,7 +686,7 @@ __global__ void k_adj_add_corr(Dev d, AdjDev a, int64_t step, double2* dst)
     const size_t nB = (size_t)d.n * Bp;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nB; e += (size_t)gridDim.x * blockDim.x) {
         const int i = (int)(e / Bp), b = (int)(e % Bp);
-        const double2 br = d.bra[i], cb = a.cbar[(size_t)step * Bp + b];
+        const double2 br = d.bra[(d.bra_idx ? (size_t)d.bra_idx[b] * d.n : 0) + i], cb = a.cbar[(size_t)step * Bp + b];
         double2 p = dst[e];
         p.x += br.x * cb.x - br.y * cb.y;
         p.y += br.x * cb.y + br.y * cb.x;

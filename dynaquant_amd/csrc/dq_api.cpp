@@ -70,6 +70,32 @@ struct RowBlocks {
     double* val64 = nullptr;
 };
 
+// LDS one workgroup of the window SpMM may use: the device's opt-in limit
+// (160 KiB on gfx950), queried once; 0 when unknown
+static size_t win_lds_limit()
+{
+    static size_t lim = [] {
+        int dev = 0, a = 0, b = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return (size_t)0;
+        if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) a = 0;
+        if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) b = 0;
+        return (size_t)std::max(a, b);
+    }();
+    return lim;
+}
+
+// Step lists of the LDS-window SpMM (dq_window.hip) for tiles of T rows and
+// slices of ts trajectories: per tile its window (first row, rows), per MFMA
+// group of G = 32 / ts four-row blocks the first step, and per batch of four
+// steps a record of 18 G dwords: the couplings [g][step][row], then the
+// uint16 window offsets [step][g], two per dword.
+struct WinLists {
+    int32_t* lo = nullptr;
+    int32_t* gptr = nullptr;
+    uint32_t* rec = nullptr;         // none for 16-row blocks (they read the rb = 16 lists)
+    int T = 0, ts = 0, R = 0, max_rows = 0, ntile = 0;
+};
+
 struct dq_system {
     dq_config cfg{};
     Dev d{};
@@ -79,6 +105,9 @@ struct dq_system {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join[kMaxSlices - 1] = {};
     int slices = 1;                   // captured steps run this many trajectory slices concurrently
+    WinLists win;                     // LDS-window SpMM lists (win.T > 0 when built)
+    int win_waves = 8;                // waves per workgroup of the window SpMM
+    bool win_lds_set = false;         // dynamic LDS limit raised for the window SpMM
     hipGraphExec_t graph = nullptr;
     double last_ms = 0.0;
     int n = 0, B = 0, Bp = 0, m = 0, K = 1;
@@ -173,6 +202,37 @@ struct dq_system {
 
 extern "C" const char* dq_last_error(void) { return g_err.c_str(); }
 
+// Device buffers a setter allocates before it replaces the old ones: released
+// again unless keep() is called, so a setter that fails half-way leaves the
+// system as it was.
+struct Pending {
+    dq_system* sys;
+    std::vector<void*> ptrs;
+    bool kept = false;
+    explicit Pending(dq_system* s) : sys(s) {}
+    template <typename T>
+    dq_status alloc(T** p, size_t count)
+    {
+        dq_status st = sys->alloc(p, count);
+        if (st == DQ_OK) ptrs.push_back((void*)*p);
+        return st;
+    }
+    void keep() { kept = true; }
+    ~Pending()
+    {
+        if (kept) return;
+        for (void* p : ptrs) sys->release(p);
+    }
+};
+
+static void release_row_blocks(dq_system* sys, RowBlocks* b)
+{
+    sys->release(b->ptr);
+    sys->release(b->col);
+    sys->release(b->val32);
+    sys->release(b->val64);
+}
+
 // the matrix-core SpMM reads x rows and dense block values through buffer
 // resources with 32-bit byte offsets: one Krylov vector (n x padded batch,
 // 8 bytes per entry) and the union slots (64 bytes each) must stay below 2^31
@@ -221,23 +281,37 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
                                         "64 * union slots must stay below 2^31 bytes");
     std::vector<float> rbv32(rbv64.size());
     for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
-    HIP_TRY(hipStreamSynchronize(sys->stream));
-    sys->release(out->ptr);
-    sys->release(out->col);
-    sys->release(out->val32);
-    sys->release(out->val64);
-    DQ_TRY(sys->alloc(&out->ptr, rbp.size()));
-    DQ_TRY(sys->alloc(&out->col, std::max<size_t>(rbc.size(), 1)));
-    DQ_TRY(sys->alloc(&out->val32, std::max<size_t>(rbv32.size(), 1)));
-    DQ_TRY(sys->alloc(&out->val64, std::max<size_t>(rbv64.size(), 1)));
-    HIP_TRY(hipMemcpyAsync(out->ptr, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
+    // the new blocks are complete before the old ones are released
+    Pending pend(sys);
+    RowBlocks nb;
+    DQ_TRY(pend.alloc(&nb.ptr, rbp.size()));
+    DQ_TRY(pend.alloc(&nb.col, std::max<size_t>(rbc.size(), 1)));
+    DQ_TRY(pend.alloc(&nb.val32, std::max<size_t>(rbv32.size(), 1)));
+    DQ_TRY(pend.alloc(&nb.val64, std::max<size_t>(rbv64.size(), 1)));
+    HIP_TRY(hipMemcpyAsync(nb.ptr, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
     if (!rbc.empty()) {
-        HIP_TRY(hipMemcpyAsync(out->col, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(out->val32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
-        HIP_TRY(hipMemcpyAsync(out->val64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(nb.col, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(nb.val32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipMemcpyAsync(nb.val64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
     }
     HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
+    release_row_blocks(sys, out);
+    *out = nb;
     return DQ_OK;
+}
+
+// The tile and slice counts every SpMM consumer uses: the row-blocked
+// kernels' tiles (4 row blocks of rb rows), and the tiles whose partial sums
+// the SpMM leaves in part_s (the window kernel's when it is on).
+static void refresh_spmm_tiles(dq_system* sys)
+{
+    Dev& d = sys->d;
+    if (!d.plain) {
+        d.rt_s = 4 * d.rb;
+        d.ntile_s = (sys->n + d.rt_s - 1) / d.rt_s;
+    }
+    d.ntile_p = (d.win_T > 0 && sys->win.lo) ? (sys->win.R == 4 ? d.win_strips : sys->win.ntile) : d.ntile_s;
 }
 
 // row blocks of R rows for the Hamiltonian (from the host CSR kept by
@@ -245,20 +319,189 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
 static dq_status build_row_blocks(dq_system* sys, int R)
 {
     Dev& d = sys->d;
-    RowBlocks a;
-    a.ptr = const_cast<int32_t*>(d.rb_ptr);
-    a.col = const_cast<int32_t*>(d.rb_col);
-    a.val32 = const_cast<float*>(d.rb_val32);
-    a.val64 = const_cast<double*>(d.rb_val64);
+    RowBlocks a, t;
     DQ_TRY(make_row_blocks(sys, R, sys->rowptr, sys->col, sys->val, &a));
-    d.rb_ptr = a.ptr; d.rb_col = a.col; d.rb_val32 = a.val32; d.rb_val64 = a.val64;
-    if (sys->tk.p > 0) DQ_TRY(make_row_blocks(sys, R, sys->tk.rowptr_h, sys->tk.col_h, sys->tk.val_h, &sys->tk.rb));
-    d.rb = R;
-    if (!d.plain) {
-        d.rt_s = 4 * R;
-        d.ntile_s = (sys->n + d.rt_s - 1) / d.rt_s;
+    if (sys->tk.p > 0) {
+        dq_status st = make_row_blocks(sys, R, sys->tk.rowptr_h, sys->tk.col_h, sys->tk.val_h, &t);
+        if (st != DQ_OK) {
+            release_row_blocks(sys, &a);
+            return st;
+        }
+        release_row_blocks(sys, &sys->tk.rb);
+        sys->tk.rb = t;
     }
+    RowBlocks old;
+    old.ptr = const_cast<int32_t*>(d.rb_ptr);
+    old.col = const_cast<int32_t*>(d.rb_col);
+    old.val32 = const_cast<float*>(d.rb_val32);
+    old.val64 = const_cast<double*>(d.rb_val64);
+    release_row_blocks(sys, &old);
+    d.rb_ptr = a.ptr; d.rb_col = a.col; d.rb_val32 = a.val32; d.rb_val64 = a.val64;
+    d.rb = R;
+    refresh_spmm_tiles(sys);
     return DQ_OK;
+}
+
+static void release_window(dq_system* sys, WinLists* w)
+{
+    sys->release(w->lo);
+    sys->release(w->gptr);
+    sys->release(w->rec);
+    *w = WinLists{};
+}
+
+// Lists of the LDS-window SpMMs (dq_window.hip) for tiles of T rows.
+//  * 16-row blocks (R = 16, slices of ts = 16 or 32 trajectories): per tile the
+//    window [lo, lo + rows) of every column its rows reference and the rows
+//    themselves; the products read the rb = 16 row-block lists.
+//  * 4-row blocks (R = 4, slices of 32 trajectories, persistent strips with an
+//    LDS ring): per tile lo' (suffix minimum of the windows' first rows) and
+//    hi' (prefix maximum of their last rows), the ring size RING = max hi' -
+//    lo' + 1, and per 4-row block its union of columns (ascending, as in the
+//    4-row blocks of make_row_blocks) in batches of four steps, each a record
+//    of 16 couplings [step][row] and the four columns' ring slots (column mod
+//    RING, uint16, two per dword); padding steps repeat the block's first
+//    column with zero couplings.
+// Fails (nothing changed) when T does not suit the blocks, or a window or the
+// ring does not fit 65,536 rows or one workgroup's LDS.
+static dq_status make_window_lists(dq_system* sys, int T, int ts, int R, WinLists* out)
+{
+    const int n = sys->n;
+    if (R != 4 && R != 16) return fail(DQ_ERR_INVALID_ARG, "window SpMM: row blocks of 4 or 16 rows");
+    if (R == 16 ? (ts != 16 && ts != 32) : ts != 32)
+        return fail(DQ_ERR_INVALID_ARG, "window SpMM: 16 or 32 trajectories per slice (32 with 4-row blocks)");
+    if (T < 16 || T % (R == 16 ? 16 : 8) != 0 || T > 4096)
+        return fail(DQ_ERR_INVALID_ARG, "window SpMM: tile rows must be a multiple of 8 (16 for 16-row blocks) "
+                                        "in [16, 4096]");
+    const int32_t* rowptr = sys->rowptr.data();
+    const int32_t* col = sys->col.data();
+    const double* val = sys->val.data();
+    const int ntile = (n + T - 1) / T;
+    std::vector<int32_t> lo(2 * (size_t)ntile), hi(ntile), gptr;
+    for (int t = 0; t < ntile; ++t) {
+        const int r0 = t * T, r1 = std::min(n, r0 + T);
+        int mn = r0, mx = r1 - 1;
+        for (int i = r0; i < r1; ++i)
+            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) { mn = std::min(mn, col[p]); mx = std::max(mx, col[p]); }
+        lo[2 * t] = mn;
+        hi[t] = mx;
+    }
+    int max_rows = 0;
+    std::vector<uint32_t> rec;
+    constexpr size_t kRec = 18;
+    if (R == 16) {
+        for (int t = 0; t < ntile; ++t) {
+            lo[2 * t + 1] = hi[t] - lo[2 * t] + 1;
+            max_rows = std::max(max_rows, lo[2 * t + 1]);
+        }
+    } else {
+        // lo' and hi', the ring and the largest number of new rows per tile
+        for (int t = ntile - 2; t >= 0; --t) lo[2 * t] = std::min(lo[2 * t], lo[2 * t + 2]);
+        for (int t = 1; t < ntile; ++t) hi[t] = std::max(hi[t], hi[t - 1]);
+        for (int t = 0; t < ntile; ++t) {
+            lo[2 * t + 1] = hi[t];
+            max_rows = std::max(max_rows, hi[t] - lo[2 * t] + 1);
+        }
+        const int ring = max_rows;
+        const int nblk = (n + 3) / 4;
+        gptr.assign((size_t)nblk + 1, 0);
+        std::vector<int32_t> u;
+        size_t steps = 0;
+        for (int rb = 0; rb < nblk; ++rb) {
+            u.clear();
+            for (int i = 4 * rb; i < std::min(n, 4 * rb + 4); ++i) u.insert(u.end(), col + rowptr[i], col + rowptr[i + 1]);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            const size_t L = (u.size() + 3) / 4 * 4;
+            rec.resize((steps + L) / 4 * kRec, 0u);
+            for (size_t k = 0; k < L; ++k) {
+                const size_t st = steps + k, b = st / 4, kk = st % 4;
+                const uint32_t slot = (uint32_t)((k < u.size() ? u[k] : u[0]) % ring);
+                rec[b * kRec + 16 + kk / 2] |= slot << (16 * (kk & 1));
+            }
+            for (int i = 4 * rb; i < std::min(n, 4 * rb + 4); ++i)
+                for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                    const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
+                    const size_t st = steps + pos, b = st / 4, kk = st % 4;
+                    const float v = (float)val[p];
+                    uint32_t bits;
+                    std::memcpy(&bits, &v, 4);
+                    rec[b * kRec + kk * 4 + (size_t)(i - 4 * rb)] = bits;
+                }
+            steps += L;
+            if (steps / 4 * kRec * 4 > (size_t)INT32_MAX) return fail(DQ_ERR_INVALID_ARG, "window SpMM: too many steps");
+            gptr[rb + 1] = (int32_t)steps;
+        }
+    }
+    if (max_rows > 65536) return fail(DQ_ERR_INVALID_ARG, "window SpMM: a window exceeds 65,536 rows");
+    const size_t lds = dq::spmm_win_lds_bytes(ts, max_rows), lds_max = win_lds_limit();
+    if (lds > lds_max)
+        return fail(DQ_ERR_INVALID_ARG, "window SpMM: the widest window needs " + std::to_string(lds) +
+                                            " bytes of LDS, more than one workgroup's " + std::to_string(lds_max));
+    rec.resize(std::max<size_t>(rec.size(), kRec), 0u);
+    gptr.resize(std::max<size_t>(gptr.size(), 1), 0);
+    WinLists w;
+    dq_status st = DQ_OK;
+    if ((st = sys->alloc(&w.lo, lo.size())) != DQ_OK || (st = sys->alloc(&w.gptr, gptr.size())) != DQ_OK ||
+        (st = sys->alloc(&w.rec, rec.size())) != DQ_OK) {
+        release_window(sys, &w);
+        return st;
+    }
+    hipError_t e = hipMemcpyAsync(w.lo, lo.data(), sizeof(int32_t) * lo.size(), hipMemcpyHostToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w.gptr, gptr.data(), sizeof(int32_t) * gptr.size(), hipMemcpyHostToDevice, sys->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w.rec, rec.data(), sizeof(uint32_t) * rec.size(), hipMemcpyHostToDevice, sys->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
+    if (e != hipSuccess) {
+        release_window(sys, &w);
+        return fail(DQ_ERR_HIP, std::string("window SpMM lists: ") + hipGetErrorString(e));
+    }
+    if (!sys->win_lds_set) {
+        e = dq::spmm_win_set_lds_limit(lds_max);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            release_window(sys, &w);
+            return fail(DQ_ERR_INVALID_ARG, std::string("window SpMM: the LDS limit cannot be raised: ") +
+                                                hipGetErrorString(e));
+        }
+        sys->win_lds_set = true;
+    }
+    w.T = T;
+    w.ts = ts;
+    w.R = R;
+    w.max_rows = max_rows;
+    w.ntile = ntile;
+    release_window(sys, out);
+    *out = w;
+    return DQ_OK;
+}
+
+// Point the device view at the window lists (T > 0) or switch the window
+// SpMM off (T = 0).
+static void attach_window(dq_system* sys)
+{
+    Dev& d = sys->d;
+    const WinLists& w = sys->win;
+    d.win_T = w.lo ? w.T : 0;
+    d.win_ts = w.ts;
+    d.win_w = sys->win_waves;
+    d.win_lds = d.win_T > 0 ? (int)dq::spmm_win_lds_bytes(w.ts, w.max_rows) : 0;
+    d.win_lo = w.lo;
+    d.win_gptr = w.gptr;
+    d.win_rec = w.rec;
+    d.win_ring = w.R == 4 ? w.max_rows : 0;
+    d.win_strips = d.win_tps = 0;
+    if (d.win_T > 0 && w.R == 4) {
+        // one persistent workgroup per CU over the whole batch: strips of
+        // consecutive tiles x slices of 32 trajectories
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        const int nsl = std::max(1, d.Bp / 32);
+        const int want = std::max(1, (cus + nsl - 1) / nsl);
+        d.win_tps = (w.ntile + want - 1) / want;
+        d.win_strips = (w.ntile + d.win_tps - 1) / d.win_tps;
+    }
+    refresh_spmm_tiles(sys);
 }
 
 extern "C" {
@@ -314,6 +557,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.rt_s = plain ? 64 : 4 * d.rb;      // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
+    d.ntile_p = d.ntile_s;
+    d.win_w = s->win_waves;
     d.S = cfg->ou_substeps;
     d.noise_model = cfg->noise_model;
     d.dt = cfg->dt_fs;
@@ -420,23 +665,18 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
             if (col[p] == i) return fail(DQ_ERR_INVALID_ARG, "diagonal entries belong in e_site");
     Dev& d = sys->d;
     sys->drop_graph();
-    sys->rowptr.assign(rowptr, rowptr + n + 1);
-    sys->col.assign(col, col + nnz);
-    sys->val.assign(val, val + nnz);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    // the new CSR, row blocks and window lists are complete before anything
+    // old is released: a failure leaves the previous Hamiltonian in place
+    Pending pend(sys);
     int32_t *rp, *cl;
     double *v64, *es;
     float* v32;
-    hipStreamSynchronize(sys->stream);
-    sys->release(*(int32_t**)&d.rowptr);
-    sys->release(*(int32_t**)&d.col);
-    sys->release(*(double**)&d.val64);
-    sys->release(*(float**)&d.val32);
-    sys->release(*(double**)&d.e_site);
-    DQ_TRY(sys->alloc(&rp, (size_t)n + 1));
-    DQ_TRY(sys->alloc(&cl, (size_t)std::max<int64_t>(nnz, 1)));
-    DQ_TRY(sys->alloc(&v64, (size_t)std::max<int64_t>(nnz, 1)));
-    DQ_TRY(sys->alloc(&v32, (size_t)std::max<int64_t>(nnz, 1)));
-    DQ_TRY(sys->alloc(&es, (size_t)n));
+    DQ_TRY(pend.alloc(&rp, (size_t)n + 1));
+    DQ_TRY(pend.alloc(&cl, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(pend.alloc(&v64, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(pend.alloc(&v32, (size_t)std::max<int64_t>(nnz, 1)));
+    DQ_TRY(pend.alloc(&es, (size_t)n));
     std::vector<float> f32(nnz);
     for (int64_t p = 0; p < nnz; ++p) f32[p] = (float)val[p];
     std::vector<double> esv(n, 0.0);
@@ -448,7 +688,40 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
         HIP_TRY(hipMemcpyAsync(v32, f32.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, sys->stream));
     }
     HIP_TRY(hipMemcpyAsync(es, esv.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
-    DQ_TRY(build_row_blocks(sys, d.rb));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    // host copies swapped in for the builders, swapped back on failure
+    std::vector<int32_t> rp_h(rowptr, rowptr + n + 1), col_h(col, col + nnz);
+    std::vector<double> val_h(val, val + nnz);
+    std::swap(sys->rowptr, rp_h);
+    std::swap(sys->col, col_h);
+    std::swap(sys->val, val_h);
+    auto restore_host = [&]() {
+        std::swap(sys->rowptr, rp_h);
+        std::swap(sys->col, col_h);
+        std::swap(sys->val, val_h);
+    };
+    WinLists nw;
+    if (sys->win.T > 0) {
+        dq_status st = make_window_lists(sys, sys->win.T, sys->win.ts, sys->win.R, &nw);
+        if (st != DQ_OK) { restore_host(); return st; }
+    }
+    dq_status st = build_row_blocks(sys, d.rb);
+    if (st != DQ_OK) {
+        release_window(sys, &nw);
+        restore_host();
+        return st;
+    }
+    pend.keep();
+    if (sys->win.T > 0) {
+        release_window(sys, &sys->win);
+        sys->win = nw;
+        attach_window(sys);
+    }
+    sys->release(*(int32_t**)&d.rowptr);
+    sys->release(*(int32_t**)&d.col);
+    sys->release(*(double**)&d.val64);
+    sys->release(*(float**)&d.val32);
+    sys->release(*(double**)&d.e_site);
     d.rowptr = rp; d.col = cl; d.val64 = v64; d.val32 = v32; d.e_site = es;
     sys->have_h = true;
     return DQ_OK;
@@ -587,18 +860,20 @@ dq_status dq_system_set_ou_params(dq_system* sys, const double* tau_fs, const do
     }
     Dev& d = sys->d;
     sys->drop_graph();
-    hipStreamSynchronize(sys->stream);
-    sys->release(*(double**)&d.inv_tau);
-    sys->release(*(double**)&d.coeff);
-    sys->release(*(double**)&d.ou_sigma);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    Pending pend(sys);
     double *a, *b, *c;
-    DQ_TRY(sys->alloc(&a, n));
-    DQ_TRY(sys->alloc(&b, n));
-    DQ_TRY(sys->alloc(&c, n));
+    DQ_TRY(pend.alloc(&a, n));
+    DQ_TRY(pend.alloc(&b, n));
+    DQ_TRY(pend.alloc(&c, n));
     HIP_TRY(hipMemcpyAsync(a, it.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipMemcpyAsync(b, co.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipMemcpyAsync(c, sg.data(), sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
+    sys->release(*(double**)&d.inv_tau);
+    sys->release(*(double**)&d.coeff);
+    sys->release(*(double**)&d.ou_sigma);
     d.inv_tau = a; d.coeff = b; d.ou_sigma = c;
     sys->have_ou = true;
     return DQ_OK;
@@ -613,11 +888,13 @@ dq_status dq_system_set_hsr_params(dq_system* sys, const double* sigma)
     Dev& d = sys->d;
     sys->drop_graph();
     HIP_TRY(hipStreamSynchronize(sys->stream));
-    sys->release(*(double**)&d.hsr_sigma);
+    Pending pend(sys);
     double* a;
-    DQ_TRY(sys->alloc(&a, n));
+    DQ_TRY(pend.alloc(&a, n));
     HIP_TRY(hipMemcpyAsync(a, sigma, sizeof(double) * n, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
+    sys->release(*(double**)&d.hsr_sigma);
     d.hsr_sigma = a;
     sys->have_hsr = true;
     return DQ_OK;
@@ -638,14 +915,16 @@ dq_status dq_system_set_zz_params(dq_system* sys, int32_t nosc, const double* om
     Dev& d = sys->d;
     sys->drop_graph();
     HIP_TRY(hipStreamSynchronize(sys->stream));
-    sys->release(*(double**)&d.zz_w2pi);
-    sys->release(*(double**)&d.zz_coeff);
+    Pending pend(sys);
     double *a, *c;
-    DQ_TRY(sys->alloc(&a, nosc));
-    DQ_TRY(sys->alloc(&c, nosc));
+    DQ_TRY(pend.alloc(&a, nosc));
+    DQ_TRY(pend.alloc(&c, nosc));
     HIP_TRY(hipMemcpyAsync(a, w.data(), sizeof(double) * nosc, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipMemcpyAsync(c, coeff, sizeof(double) * nosc, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
+    sys->release(*(double**)&d.zz_w2pi);
+    sys->release(*(double**)&d.zz_coeff);
     d.zz_w2pi = a;
     d.zz_coeff = c;
     d.zz_nosc = nosc;
@@ -727,24 +1006,32 @@ dq_status dq_system_reset_disorder(dq_system* sys)
     return DQ_OK;
 }
 
+// The reference normalises every initial state on the host
+// (systemClass.cu:1383-1481, normalizeHostCol).  Here |psi_b|^2 of each
+// trajectory's active sites is handed to the device as the norm the first
+// prologue divides by, the same renormalisation every later step applies.
 dq_status dq_system_set_state(dq_system* sys, const double* psi)
 {
     if (!sys || !psi) return fail(DQ_ERR_INVALID_ARG, "null argument");
     Dev& d = sys->d;
     const int n = sys->n, Bp = d.Bp;
-    std::vector<double> h((size_t)2 * n * Bp, 0.0);
+    std::vector<double> h((size_t)2 * n * Bp, 0.0), n2(Bp, 1.0);
     for (int b = 0; b < sys->B; ++b) {
         const int sb = sys->slot[b], na = sys->nact_slot[sb];
+        double s2 = 0.0;
         for (int i = 0; i < n; ++i) {
             const double re = psi[((size_t)b * n + i) * 2], im = psi[((size_t)b * n + i) * 2 + 1];
             if (!std::isfinite(re) || !std::isfinite(im)) return fail(DQ_ERR_NONFINITE, "non-finite state");
             if (i >= na) continue;                 // sites beyond n_active are not part of b
             h[((size_t)i * Bp + sb) * 2] = re;
             h[((size_t)i * Bp + sb) * 2 + 1] = im;
+            s2 += re * re + im * im;
         }
+        if (!(s2 > 0.0)) return fail(DQ_ERR_INVALID_ARG, "trajectory " + std::to_string(b) + ": zero state on its active sites");
+        n2[sb] = s2;
     }
     HIP_TRY(hipMemcpyAsync(d.psi, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, sys->stream));
-    HIP_TRY(dq::launch_set_unit_norm(d, sys->stream));
+    HIP_TRY(hipMemcpyAsync(d.nrm2, n2.data(), sizeof(double) * Bp, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipStreamSynchronize(sys->stream));
     sys->have_state = true;
     return DQ_OK;
@@ -768,6 +1055,90 @@ dq_status dq_system_get_state(dq_system* sys, double* psi)
     return DQ_OK;
 }
 
+// Resume point of a run (no counterpart in the reference, whose runs restart
+// from the initial state): the state psi as the next step will use it, the
+// OU offsets, the error sums and the step counter.  Together with the
+// deterministic static disorder (dq_system_reset_disorder) and the Philox
+// counters of the noise, they fix every later step bit for bit.
+dq_status dq_system_snapshot(dq_system* sys, double* psi, double* noise, double* err_sum, int64_t* step)
+{
+    if (!sys || !psi || !noise || !step) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (!sys->have_state) return fail(DQ_ERR_STATE, "no state to snapshot (call dq_system_set_state)");
+    Dev& d = sys->d;
+    const int n = sys->n, Bp = d.Bp;
+    // a normalisation still pending from dq_system_set_state is applied now,
+    // exactly as the next step's prologue would (afterwards that is a no-op)
+    HIP_TRY(dq::launch_prologue(d, 1, 0, 0, 0, sys->stream));
+    HIP_TRY(dq::launch_set_unit_norm(d, sys->stream));
+    DQ_TRY(dq_system_get_state(sys, psi));
+    std::vector<double> h((size_t)n * Bp);
+    HIP_TRY(hipMemcpy(h.data(), d.de, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    for (int b = 0; b < sys->B; ++b)
+        for (int i = 0; i < n; ++i) noise[(size_t)b * n + i] = h[(size_t)i * Bp + sys->slot[b]];
+    if (err_sum) DQ_TRY(dq_get_error_sum(sys, err_sum));
+    int64_t s = 0;
+    HIP_TRY(hipMemcpy(&s, d.step, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *step = s;
+    return DQ_OK;
+}
+
+dq_status dq_system_restore(dq_system* sys, const double* psi, const double* noise, const double* err_sum,
+                            int64_t step)
+{
+    if (!sys || !psi || !noise) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (!sys->have_h || !sys->noise_ready() || !sys->have_disorder)
+        return fail(DQ_ERR_STATE, "set the Hamiltonian and noise parameters and call dq_system_reset_disorder first");
+    if (step < 0) return fail(DQ_ERR_INVALID_ARG, "step < 0");
+    Dev& d = sys->d;
+    const int n = sys->n, Bp = d.Bp;
+    const size_t nB = (size_t)n * Bp;
+    std::vector<double> hp(2 * nB, 0.0), hd(nB, 0.0), e0(nB), es(Bp, 0.0), n2(Bp, 1.0);
+    for (int b = 0; b < sys->B; ++b) {
+        const int sb = sys->slot[b], na = sys->nact_slot[sb];
+        for (int i = 0; i < na; ++i) {
+            const double re = psi[((size_t)b * n + i) * 2], im = psi[((size_t)b * n + i) * 2 + 1];
+            const double de = noise[(size_t)b * n + i];
+            if (!std::isfinite(re) || !std::isfinite(im) || !std::isfinite(de))
+                return fail(DQ_ERR_NONFINITE, "non-finite snapshot");
+            hp[((size_t)i * Bp + sb) * 2] = re;
+            hp[((size_t)i * Bp + sb) * 2 + 1] = im;
+            hd[(size_t)i * Bp + sb] = de;
+        }
+        if (err_sum) es[sb] = err_sum[b];
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(e0.data(), d.e0, sizeof(double) * nB, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpyAsync(d.psi, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(d.nrm2, n2.data(), sizeof(double) * Bp, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(d.de, hd.data(), sizeof(double) * nB, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(d.err_sum, es.data(), sizeof(double) * Bp, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(d.step, &step, sizeof(int64_t), hipMemcpyHostToDevice, sys->stream));
+    // the diagonal the last noise update wrote: static energy + offset, in the
+    // basis precision (active sites; the rest as reset_disorder left them)
+    if (d.mixed) {
+        std::vector<float> dg(nB);
+        HIP_TRY(hipMemcpy(dg.data(), d.diag, sizeof(float) * nB, hipMemcpyDeviceToHost));
+        for (int b = 0; b < sys->B; ++b) {
+            const int sb = sys->slot[b], na = sys->nact_slot[sb];
+            for (int i = 0; i < na; ++i) dg[(size_t)i * Bp + sb] = (float)(hd[(size_t)i * Bp + sb] + e0[(size_t)i * Bp + sb]);
+        }
+        HIP_TRY(hipMemcpyAsync(d.diag, dg.data(), sizeof(float) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+    } else {
+        std::vector<double> dg(nB);
+        HIP_TRY(hipMemcpy(dg.data(), d.diag, sizeof(double) * nB, hipMemcpyDeviceToHost));
+        for (int b = 0; b < sys->B; ++b) {
+            const int sb = sys->slot[b], na = sys->nact_slot[sb];
+            for (int i = 0; i < na; ++i) dg[(size_t)i * Bp + sb] = hd[(size_t)i * Bp + sb] + e0[(size_t)i * Bp + sb];
+        }
+        HIP_TRY(hipMemcpyAsync(d.diag, dg.data(), sizeof(double) * nB, hipMemcpyHostToDevice, sys->stream));
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+    }
+    sys->step = step;
+    sys->have_state = true;
+    return DQ_OK;
+}
+
 dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_t ngroups, int32_t every,
                                     int64_t max_steps, const double* bra)
 {
@@ -786,28 +1157,34 @@ dq_status dq_system_set_observables(dq_system* sys, const int32_t* group, int32_
             if (group[i] >= ngroups) return fail(DQ_ERR_INVALID_ARG, "group index out of range");
     }
     sys->drop_graph();
-    hipStreamSynchronize(sys->stream);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    Pending pend(sys);
+    const int64_t nrec = every > 0 ? max_steps / every + 1 : 0;
+    int32_t* gp = nullptr;
+    double* pops = nullptr;
+    double2 *br = nullptr, *corr = nullptr;
+    if (every > 0) {
+        DQ_TRY(pend.alloc(&gp, n));
+        DQ_TRY(pend.alloc(&pops, (size_t)nrec * (site_mode ? n : ngroups) * d.Bp));
+        HIP_TRY(hipMemcpyAsync(gp, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, sys->stream));
+    }
+    if (bra) {
+        DQ_TRY(pend.alloc(&br, n));
+        DQ_TRY(pend.alloc(&corr, (size_t)(max_steps + 1) * d.Bp));
+        HIP_TRY(hipMemcpyAsync(br, bra, sizeof(double2) * n, hipMemcpyHostToDevice, sys->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
     sys->release(*(int32_t**)&d.group);
     sys->release(d.pops);
     sys->release(*(double2**)&d.bra);
+    sys->release(*(int32_t**)&d.bra_idx);
     sys->release(d.corr);
     sys->release(d.partic);
-    const int64_t nrec = every > 0 ? max_steps / every + 1 : 0;
-    if (every > 0) {
-        int32_t* gp;
-        DQ_TRY(sys->alloc(&gp, n));
-        HIP_TRY(hipMemcpyAsync(gp, group, sizeof(int32_t) * n, hipMemcpyHostToDevice, sys->stream));
-        d.group = gp;
-        DQ_TRY(sys->alloc(&d.pops, (size_t)nrec * (site_mode ? n : ngroups) * d.Bp));
-    }
-    if (bra) {
-        double2* br;
-        DQ_TRY(sys->alloc(&br, n));
-        HIP_TRY(hipMemcpyAsync(br, bra, sizeof(double2) * n, hipMemcpyHostToDevice, sys->stream));
-        d.bra = br;
-        DQ_TRY(sys->alloc(&d.corr, (size_t)(max_steps + 1) * d.Bp));
-    }
-    HIP_TRY(hipStreamSynchronize(sys->stream));
+    d.group = gp;
+    d.pops = pops;
+    d.bra = br;
+    d.corr = corr;
     d.ngroups = ngroups;
     d.every = every;
     d.site_mode = site_mode;
@@ -913,6 +1290,9 @@ static Dev tk_dev(const dq_system* sys, const Dev& d)
     a.rb_val64 = sys->tk.rb.val64;
     a.diag = sys->tk.diag;
     a.g = 1.0;
+    // the auxiliary matrix goes through the row-blocked SpMM
+    a.win_T = 0;
+    a.ntile_p = a.ntile_s;
     return a;
 }
 
@@ -1259,36 +1639,105 @@ dq_status dq_fft(int32_t n, int32_t batch, double* data, int32_t sign)
     return DQ_OK;
 }
 
-dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft, double* omega, double* spec)
+}  // extern "C"
+
+// Spectrum of C(n) = (wts ? sum_b wts[b] C_b(n) : batch mean) times scale / T:
+// window, zero padding to nfft, inverse-direction FFT on the GPU
+// (spectraClass.cu:575-750).  wts_host: per caller trajectory, or null.
+static dq_status spectrum_impl(dq_system* sys, double sigma_fs, double scale_mu2, const double* wts_host,
+                               int32_t nfft, double* omega, double* spec)
 {
     if (!sys || !omega || !spec) return fail(DQ_ERR_INVALID_ARG, "null argument");
     if (!sys->have_bra) return fail(DQ_ERR_STATE, "no correlation function (bra not set)");
     if (nfft < 2 || (nfft & (nfft - 1))) return fail(DQ_ERR_INVALID_ARG, "nfft must be a power of two");
     const int64_t T = std::min(sys->step, sys->max_steps);
     if (T < 1 || T + 1 > nfft) return fail(DQ_ERR_INVALID_ARG, "need 1 <= steps < nfft");
+    std::vector<double> wslot;
+    if (wts_host) {
+        wslot.assign(sys->d.Bp, 0.0);
+        for (int b = 0; b < sys->B; ++b) {
+            if (!std::isfinite(wts_host[b])) return fail(DQ_ERR_NONFINITE, "non-finite spectrum weight");
+            wslot[sys->slot[b]] = wts_host[b];
+        }
+    }
     double2 *a = nullptr, *b = nullptr;
+    double* w = nullptr;
     const size_t bytes = sizeof(double2) * (size_t)nfft;
     hipError_t e;
-    if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess) {
-        hipFree(a); hipFree(b);
+    if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess ||
+        (wts_host && (e = hipMalloc(&w, sizeof(double) * wslot.size())) != hipSuccess)) {
+        hipFree(a); hipFree(b); hipFree(w);
         return fail(DQ_ERR_OOM, hipGetErrorString(e));
     }
-    e = dq::launch_spectrum_prep(sys->d, T, sys->cfg.dt_fs, sigma_fs, nfft, a, sys->stream);
+    if (w) e = hipMemcpyAsync(w, wslot.data(), sizeof(double) * wslot.size(), hipMemcpyHostToDevice, sys->stream);
+    if (e == hipSuccess) e = dq::launch_spectrum_prep(sys->d, T, sys->cfg.dt_fs, sigma_fs, nfft, a, w, sys->stream);
     if (e == hipSuccess) e = dq::launch_fft(nfft, 1, a, b, +1, sys->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
     if (e == hipSuccess) e = hipMemcpy(spec, a, bytes, hipMemcpyDeviceToHost);
-    hipFree(a); hipFree(b);
+    hipFree(a); hipFree(b); hipFree(w);
     if (e != hipSuccess) return fail(DQ_ERR_HIP, hipGetErrorString(e));
     // spectraClass.cu:147-151 (true lightspeed on the frequency axis) and
     // the |mu|^2 / T scaling of spectraClass.cu:641-686
     const double inv_fs_to_inv_cm = 1. / (299792458. * 100. * 1.e-15);
-    const double scale = mu2 / (double)T;
+    const double scale = scale_mu2 / (double)T;
     for (int k = 0; k < nfft; ++k) {
         const int kk = k > nfft / 2 ? k - nfft : k;
         omega[k] = inv_fs_to_inv_cm * (double)kk / (sys->cfg.dt_fs * (double)nfft);
         spec[2 * k] *= scale;
         spec[2 * k + 1] *= scale;
     }
+    return DQ_OK;
+}
+
+extern "C" {
+
+dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft, double* omega, double* spec)
+{
+    return spectrum_impl(sys, sigma_fs, mu2, nullptr, nfft, omega, spec);
+}
+
+dq_status dq_spectrum_weighted(dq_system* sys, double sigma_fs, const double* weight, int32_t nfft, double* omega,
+                               double* spec)
+{
+    if (!weight) return fail(DQ_ERR_INVALID_ARG, "null weights");
+    return spectrum_impl(sys, sigma_fs, 1.0, weight, nfft, omega, spec);
+}
+
+// Several bras: trajectory b records C_b(n) = <bras[bra_of_traj[b]]|psi_b(n)>
+// (the reference runs one ensemble per dipole direction, each with its own
+// bra, spectraClass.cu:437-461, 699-744)
+dq_status dq_system_set_bras(dq_system* sys, int32_t nbra, const double* bras, const int32_t* bra_of_traj)
+{
+    if (!sys || !bras || !bra_of_traj) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (nbra < 1) return fail(DQ_ERR_INVALID_ARG, "nbra must be >= 1");
+    Dev& d = sys->d;
+    const int n = sys->n;
+    for (size_t q = 0; q < (size_t)2 * n * nbra; ++q)
+        if (!std::isfinite(bras[q])) return fail(DQ_ERR_NONFINITE, "non-finite bra");
+    std::vector<int32_t> idx(d.Bp, 0);
+    for (int b = 0; b < sys->B; ++b) {
+        if (bra_of_traj[b] < 0 || bra_of_traj[b] >= nbra) return fail(DQ_ERR_INVALID_ARG, "bra index out of range");
+        idx[sys->slot[b]] = bra_of_traj[b];
+    }
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    Pending pend(sys);
+    double2* br = nullptr;
+    int32_t* bi = nullptr;
+    double2* corr = nullptr;
+    DQ_TRY(pend.alloc(&br, (size_t)n * nbra));
+    DQ_TRY(pend.alloc(&bi, (size_t)d.Bp));
+    if (!d.corr) DQ_TRY(pend.alloc(&corr, (size_t)(sys->max_steps + 1) * d.Bp));
+    HIP_TRY(hipMemcpyAsync(br, bras, sizeof(double2) * n * nbra, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(bi, idx.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    pend.keep();
+    sys->release(*(double2**)&d.bra);
+    sys->release(*(int32_t**)&d.bra_idx);
+    d.bra = br;
+    d.bra_idx = bi;
+    if (corr) d.corr = corr;
+    sys->have_bra = true;
     return DQ_OK;
 }
 
@@ -1317,7 +1766,9 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     ckpt = (int32_t)std::min<int64_t>(ckpt, nsteps);
     const int ncyc = ckpt * K;
     const int64_t nck = (nsteps + ckpt - 1) / ckpt;
-    const int max_s = 24;
+    // the forward Pade kernel accepts s < 50 (padeClass.cu:618-623); the
+    // reverse tapes every squaring, so its scratch is sized for the same bound
+    const int max_s = 49;
     std::vector<void*> tmp;
     auto get = [&](void** p, size_t bytes) -> dq_status {
         hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 16);
@@ -1326,13 +1777,23 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
         hipMemsetAsync(*p, 0, bytes > 0 ? bytes : 16, st);
         return DQ_OK;
     };
-    auto cleanup = [&]() { hipStreamSynchronize(st); for (void* p : tmp) hipFree(p); };
+    // every exit, error paths included, frees the adjoint's buffers
+    struct Scratch {
+        hipStream_t st;
+        std::vector<void*>& tmp;
+        ~Scratch()
+        {
+            hipStreamSynchronize(st);
+            for (void* p : tmp) hipFree(p);
+            tmp.clear();
+        }
+    } scratch_guard{st, tmp};
     dq::AdjDev a{};
     a.m = m; a.K = K; a.nB = nB; a.vs = d.vs; a.Bp = Bp; a.max_s = max_s;
     double2 *ck_psi, *ad_pbar;
     double *ck_de, *ck_nrm;
     dq_status rc = DQ_OK;
-#define AGET(p, bytes) do { if ((rc = get((void**)(p), (bytes))) != DQ_OK) { cleanup(); return rc; } } while (0)
+#define AGET(p, bytes) do { if ((rc = get((void**)(p), (bytes))) != DQ_OK) return rc; } while (0)
     AGET(&ck_psi, sizeof(double2) * nB * nck);
     AGET(&ck_de, sizeof(double) * nB * nck);
     AGET(&ck_nrm, sizeof(double) * Bp * nck);
@@ -1343,6 +1804,12 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     AGET(&a.nrm2, sizeof(double) * Bp * ncyc);
     AGET(&a.tdiag, sizeof(double) * nB * ckpt);
     AGET(&a.final_psi, sizeof(double2) * nB);
+    // the forward run's final noise offsets, diagonal and error sums, put back
+    // after the reverse sweep has recomputed earlier segments over them
+    double *fin_de, *fin_diag, *fin_err;
+    AGET(&fin_de, sizeof(double) * nB);
+    AGET(&fin_diag, sizeof(double) * nB);
+    AGET(&fin_err, sizeof(double) * Bp);
     AGET(&a.pbar, sizeof(double2) * nB);
     AGET(&a.pbar1, sizeof(double2) * nB);
     AGET(&a.wb, sizeof(double2) * nB);
@@ -1392,6 +1859,12 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
     }
     HIP_TRY(enqueue_finalize(sys));
     HIP_TRY(hipMemcpyAsync(a.final_psi, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(fin_de, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+    // the taped last segment wrote its diagonals into the tape: the final
+    // step's is the last of them
+    HIP_TRY(hipMemcpyAsync(fin_diag, a.tdiag + (size_t)(nsteps - 1 - last_s0) * nB, sizeof(double) * nB,
+                           hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(fin_err, d.err_sum, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     sys->step = nsteps;
     DQ_TRY(dq_sync(sys));
@@ -1477,12 +1950,15 @@ dq_status dq_adjoint(dq_system* sys, int64_t nsteps, int32_t ckpt, const double*
         for (int t = 0; t < d.ntile_s; ++t) g += gp[(size_t)t * Bp + sb];
         grad_g[b] = g;
     }
-    // leave the system at the final state of the forward run
+    // leave the system at the final state of the forward run: psi, noise
+    // offsets, diagonal, error sums and step counter
     HIP_TRY(hipMemcpyAsync(d.psi, hpsi.data(), sizeof(double2) * nB, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d.de, fin_de, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d.diag, fin_diag, sizeof(double) * nB, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d.err_sum, fin_err, sizeof(double) * Bp, hipMemcpyDeviceToDevice, st));
     int64_t sT = nsteps;
     HIP_TRY(hipMemcpyAsync(d.step, &sT, sizeof(int64_t), hipMemcpyHostToDevice, st));
     HIP_TRY(dq::launch_set_unit_norm(d, st));
-    cleanup();
 #undef AGET
     return DQ_OK;
 }
@@ -1523,7 +1999,8 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "spmm_traj_per_lane must be 1 or 2";
         return false;
     }
-    if (t.spmm_rows_per_wave != 16 && t.spmm_traj_per_lane == 2 && !(d.mixed && d.Bp % 128 == 0)) {
+    if (t.spmm_rows_per_wave != 16 && t.spmm_window_rows == 0 && t.spmm_traj_per_lane == 2 &&
+        !(d.mixed && d.Bp % 128 == 0)) {
         *why = "spmm_traj_per_lane = 2 needs mixed precision and a padded batch that is a multiple of 128";
         return false;
     }
@@ -1549,6 +2026,35 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "concurrent_slices needs slices of a multiple of 64 trajectory slots";
         return false;
     }
+    if (t.spmm_window_rows < 0) {
+        *why = "spmm_window_rows must be >= 0";
+        return false;
+    }
+    if (t.spmm_window_rows > 0) {
+        if (!d.mixed) {
+            *why = "the LDS-window SpMM (spmm_window_rows > 0) is for mixed precision";
+            return false;
+        }
+        if ((t.spmm_rows_per_wave != 4 && t.spmm_rows_per_wave != 16) ||
+            (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) ||
+            (t.spmm_rows_per_wave == 4 && t.spmm_traj_per_lane != 2) || t.spmm_phases != 1 ||
+            t.spmm_pipeline != 0) {
+            *why = "the LDS-window SpMM needs spmm_rows_per_wave 16 with spmm_traj_per_lane 1 or 2 (16 or 32 "
+                   "trajectories per slice) or 4 with spmm_traj_per_lane 2, spmm_phases = 1 and "
+                   "spmm_pipeline = 0";
+            return false;
+        }
+        const int mult = t.spmm_rows_per_wave == 16 ? 16 : 8;
+        if (t.spmm_window_rows % mult != 0 || t.spmm_window_rows < 16 || t.spmm_window_rows > 4096) {
+            *why = "spmm_window_rows must be a multiple of 8 (16 with 16-row blocks) in [16, 4096]";
+            return false;
+        }
+        if (t.spmm_window_waves != 0 && t.spmm_window_waves != 4 && t.spmm_window_waves != 8 &&
+            t.spmm_window_waves != 16) {
+            *why = "spmm_window_waves must be 4, 8 or 16 (0: 8)";
+            return false;
+        }
+    }
     return true;
 }
 
@@ -1561,26 +2067,53 @@ static void get_tuning(const dq_system* sys, dq_tuning* t)
     t->tile_rows = sys->d.plain ? 0 : sys->d.rt_e;
     t->spmm_pipeline = sys->d.spmm_pipe;
     t->concurrent_slices = sys->slices;
+    t->spmm_window_rows = sys->win.T;
+    t->spmm_window_waves = sys->win.T > 0 ? sys->win_waves : 0;
 }
 
+// Applies a validated tuning; the row blocks and window lists it needs are
+// built before anything is replaced, so a failure (a window wider than one
+// workgroup's LDS) leaves the previous choice in place.
 static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
 {
     Dev& d = sys->d;
+    const int T = t.spmm_window_rows, ts = 16 * t.spmm_traj_per_lane;
+    WinLists nw;
+    const int R = t.spmm_rows_per_wave;
+    const bool new_win =
+        T > 0 && sys->have_h && (sys->win.T != T || sys->win.ts != ts || sys->win.R != R || !sys->win.lo);
+    if (new_win) DQ_TRY(make_window_lists(sys, T, ts, R, &nw));
     if (t.spmm_rows_per_wave != d.rb) {
         if (sys->have_h) {
-            DQ_TRY(build_row_blocks(sys, t.spmm_rows_per_wave));
+            dq_status st = build_row_blocks(sys, t.spmm_rows_per_wave);
+            if (st != DQ_OK) {
+                release_window(sys, &nw);
+                return st;
+            }
         } else {
             d.rb = t.spmm_rows_per_wave;
-            d.rt_s = 4 * d.rb;
-            d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
         }
     }
+    if (T == 0) {
+        release_window(sys, &sys->win);
+    } else if (new_win) {
+        release_window(sys, &sys->win);
+        sys->win = nw;
+    } else if (!sys->have_h) {
+        // built by set_hamiltonian
+        release_window(sys, &sys->win);
+        sys->win.T = T;
+        sys->win.ts = ts;
+        sys->win.R = R;
+    }
+    sys->win_waves = t.spmm_window_waves > 0 ? t.spmm_window_waves : 8;
     d.spmm_tpl = t.spmm_traj_per_lane;
     d.spmm_phases = t.spmm_phases;
     d.spmm_pipe = t.spmm_pipeline;
     sys->slices = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
     d.rt_e = t.tile_rows;
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
+    attach_window(sys);
     return DQ_OK;
 }
 
@@ -1697,8 +2230,9 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 dq_tuning t;
                 std::memset(&t, 0, sizeof(t));
                 std::string why;
-                if (std::sscanf(line + key.size(), "%d %d %d %d %d %d", &t.spmm_rows_per_wave, &t.spmm_traj_per_lane,
-                                &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline, &t.concurrent_slices) == 6 &&
+                if (std::sscanf(line + key.size(), "%d %d %d %d %d %d %d %d", &t.spmm_rows_per_wave,
+                                &t.spmm_traj_per_lane, &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline,
+                                &t.concurrent_slices, &t.spmm_window_rows, &t.spmm_window_waves) == 8 &&
                     tuning_valid(sys, t, &why)) {
                     best = t;
                     hit = true;
@@ -1730,11 +2264,14 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(saved_de, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(saved_diag, d.diag, diag_bytes, hipMemcpyDeviceToDevice, sys->stream);
-    for (int R : {4, 2, 8, 16}) {
-        for (int tpl : {2, 1, 4}) {
-            for (int ph : {1, 2, 4}) {
+    // candidates: the row-blocked SpMMs, then (mixed precision) the
+    // LDS-window SpMM, grouped so that each list is built once
+    std::vector<dq_tuning> cands;
+    static const int rows[4] = {128, 64, 256, 32};
+    for (int R : {4, 2, 8, 16})
+        for (int tpl : {2, 1, 4})
+            for (int ph : {1, 2, 4})
                 for (int cand = 0; cand < 8; ++cand) {
-                    if (e != hipSuccess || st != DQ_OK) break;
                     // the matrix-core SpMM has no phase or pipeline variants
                     if (R == 16 && (ph != 1 || cand >= 4)) continue;
                     if (R != 16 && tpl == 4) continue;
@@ -1743,28 +2280,47 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                     t.spmm_rows_per_wave = R;
                     t.spmm_traj_per_lane = tpl;
                     t.spmm_phases = ph;
-                    static const int rows[4] = {128, 64, 256, 32};
                     t.tile_rows = rows[cand % 4];
                     t.spmm_pipeline = cand / 4;
-                    std::string why;
-                    if (!tuning_valid(sys, t, &why)) continue;
-                    if ((st = apply_tuning(sys, t)) != DQ_OK) break;
-                    if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;       // warm-up
-                    float ms_min = 1e30f;
-                    for (int r = 0; r < reps && e == hipSuccess; ++r) {
-                        float ms = 0.f;
-                        if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
-                        if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;
-                        if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
-                        if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
-                        if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
-                        ms_min = std::min(ms_min, ms);
-                    }
-                    if (e == hipSuccess) timed_choices.push_back({ms_min, t});
-                    if (e == hipSuccess && ms_min < best_ms) { best_ms = ms_min; best = t; }
+                    cands.push_back(t);
                 }
-            }
+    if (d.mixed)
+        for (int R : {16, 4})
+        for (int T : {240, 120, 64, 128, 256})
+            for (int tpl : {2, 1})
+                for (int W : {8, 4, 16})
+                    for (int cand = 0; cand < 4; ++cand) {
+                        if ((R == 16 && T % 16) || (R == 4 && tpl != 2)) continue;
+                        dq_tuning t;
+                        std::memset(&t, 0, sizeof(t));
+                        t.spmm_rows_per_wave = R;
+                        t.spmm_traj_per_lane = tpl;
+                        t.spmm_phases = 1;
+                        t.tile_rows = rows[cand];
+                        t.spmm_window_rows = T;
+                        t.spmm_window_waves = W;
+                        cands.push_back(t);
+                    }
+    for (const dq_tuning& t : cands) {
+        if (e != hipSuccess || st != DQ_OK) break;
+        std::string why;
+        if (!tuning_valid(sys, t, &why)) continue;
+        const dq_status sa = apply_tuning(sys, t);
+        if (sa == DQ_ERR_INVALID_ARG && t.spmm_window_rows > 0) continue;   // window wider than the LDS
+        if ((st = sa) != DQ_OK) break;
+        if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;       // warm-up
+        float ms_min = 1e30f;
+        for (int r = 0; r < reps && e == hipSuccess; ++r) {
+            float ms = 0.f;
+            if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
+            if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;
+            if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
+            if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
+            if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
+            ms_min = std::min(ms_min, ms);
         }
+        if (e == hipSuccess) timed_choices.push_back({ms_min, t});
+        if (e == hipSuccess && ms_min < best_ms) { best_ms = ms_min; best = t; }
     }
     // concurrent slices: the four fastest kernel choices of one stream, each
     // captured as graphs of 1, 2 and 4 slices (a choice that is fastest alone
@@ -1814,8 +2370,9 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     DQ_TRY(apply_tuning(sys, best));
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave, best.spmm_traj_per_lane,
-                         best.spmm_phases, best.tile_rows, best.spmm_pipeline, best.concurrent_slices);
+            std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave,
+                         best.spmm_traj_per_lane, best.spmm_phases, best.tile_rows, best.spmm_pipeline,
+                         best.concurrent_slices, best.spmm_window_rows, best.spmm_window_waves);
             std::fclose(f);
         }
     }

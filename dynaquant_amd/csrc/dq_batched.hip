@@ -528,6 +528,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
 
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
+    if (d.win_T > 0) return launch_spmm_win(d, j, last, s);
     if (d.rb == 16) {
         // the matrix-core kernel (mixed precision only, enforced by the tuning
         // checks); spmm_traj_per_lane 1, 2, 4 = 16, 32, 64 trajectories per wave
@@ -786,7 +787,7 @@ __global__ __launch_bounds__(1024) void k_reduce_all(Dev d, int j)
     const int b = d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     const int Bp = d.Bp;
     if (y < 2)
-        wide_tile_sum(d.part_s + (size_t)y * d.ntile_s * Bp + b, d.ntile_s, Bp,
+        wide_tile_sum(d.part_s + (size_t)y * d.ntile_p * Bp + b, d.ntile_p, Bp,
                       d.red + ((size_t)kRedC(d.m) + y) * Bp + b);
     else
         wide_tile_sum(d.part + (size_t)(y - 2) * d.ntile_e * Bp + b, d.ntile_e, Bp, d.red + (size_t)(y - 2) * Bp + b);

@@ -16,6 +16,7 @@ constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
 constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
 constexpr int kRowsMin = 32;       // smallest elementwise tile (rows per workgroup)
+constexpr int kRollPre = 8;        // ring SpMM: float4 per thread held for the next tile's rows
 
 struct Dev {
     int n, Bp, B, m;
@@ -23,6 +24,8 @@ struct Dev {
     int plain;                      // 1: B == 1 single-trajectory kernels
     int rt_e, ntile_e;              // elementwise kernels: rows per tile, tiles
     int rt_s, ntile_s;              // SpMM kernels
+    int ntile_p;                    // tiles of the SpMM partials in part_s (ntile_s, or the
+                                    // window kernel's tiles when win_T > 0)
     int rb;                         // batched SpMM: rows per wavefront (row block)
     int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
     int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
@@ -51,6 +54,14 @@ struct Dev {
     const int32_t* rb_col;          // [total union columns]
     const float* rb_val32;          // [total][rb]
     const double* rb_val64;         // [total][rb]
+    // LDS-window SpMM (dq_window.hip), mixed precision: tiles of win_T rows,
+    // slices of win_ts trajectories, win_w waves per workgroup
+    int win_T, win_ts, win_w, win_lds;
+    int win_ring, win_strips, win_tps;  // 4-row blocks: LDS ring rows, strips, tiles per strip
+    const int32_t* win_lo;          // [tile][2] 16-row blocks: first window row, window rows;
+                                    // 4-row blocks: lo' (suffix minimum), hi' (prefix maximum)
+    const int32_t* win_gptr;        // [row block + 1] first step of each 4-row block
+    const uint32_t* win_rec;        // [step / 4][18] batch records: couplings, uint16 ring slots
     const double* inv_tau;          // [n]
     const double* coeff;            // [n] sigma*sqrt(2/tau)
     const double* ou_sigma;         // [n]
@@ -87,7 +98,8 @@ struct Dev {
     int ngroups, site_mode, every;
     int64_t max_steps;
     double* pops;                   // [rec][g][Bp] or [rec][i][Bp] (site mode)
-    const double2* bra;             // [n] or null
+    const double2* bra;             // [nbra][n] or null
+    const int32_t* bra_idx;         // [Bp] row of the bra table of each slot, or null (row 0)
     double2* corr;                  // [max_steps+1][Bp]
     double* partic;                 // [max_steps+1][Bp] participation ratio, or null
     // Taylor-Krylov propagation (taylorKrylovClass.cu:133-434)
@@ -129,6 +141,10 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
 // (part, 2j+1 quantities) into red[0..2j], in one launch
 hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
+// LDS-window SpMM (dq_window.hip)
+hipError_t launch_spmm_win(const Dev& d, int j, int last, hipStream_t s);
+size_t spmm_win_lds_bytes(int ts, int max_rows);
+hipError_t spmm_win_set_lds_limit(size_t bytes);
 hipError_t launch_combine_b(const Dev& d, hipStream_t s);
 hipError_t launch_pade(const Dev& d, hipStream_t s);
 hipError_t launch_combine(const Dev& d, hipStream_t s);
@@ -145,6 +161,6 @@ hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s);
 hipError_t launch_tk_start(const Dev& d, hipStream_t s);
 hipError_t launch_tk_combine(const Dev& d, hipStream_t s);
 hipError_t launch_spectrum_prep(const Dev& d, int64_t T, double dt, double sigma, int nfft,
-                                double2* buf, hipStream_t s);
+                                double2* buf, const double* wts, hipStream_t s);
 
 }  // namespace dq

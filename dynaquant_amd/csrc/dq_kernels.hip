@@ -217,6 +217,8 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
     }
     const int nact = d.nact[b];
     const uint64_t tid_b = d.traj_id[b];
+    // this trajectory's bra (a table of bras, e.g. one per dipole direction)
+    const double2* __restrict__ brow = corr_on ? d.bra + (d.bra_idx ? (size_t)d.bra_idx[b] * d.n : 0) : nullptr;
     double acc[kMaxGroups + 3];
 #pragma unroll
     for (int q = 0; q < kMaxGroups + 3; ++q) acc[q] = 0.0;
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
             }
         }
         if (corr_on) {
-            double2 br = d.bra[i];
+            double2 br = brow[i];
             acc[kMaxGroups] += br.x * p.x + br.y * p.y;
             acc[kMaxGroups + 1] += br.x * p.y - br.y * p.x;
         }
@@ -868,7 +870,7 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
     {
         const double* ps = d.plain ? d.part : d.part_s;
         double s = 0.0, v = 0.0;
-        for (int t = threadIdx.x; t < d.ntile_s; t += kThreads) s += ps[(size_t)t * Bp + b];
+        for (int t = threadIdx.x; t < d.ntile_p; t += kThreads) s += ps[(size_t)t * Bp + b];
         if (!d.plain)
             for (int t = threadIdx.x; t < d.ntile_e; t += kThreads)
                 v += d.part[((size_t)(2 * m) * d.ntile_e + t) * Bp + b];
@@ -1279,18 +1281,26 @@ hipError_t launch_tk_combine(const Dev& d, hipStream_t s)
 // zero-pad to nfft.  Then a radix-2 Stockham autosort FFT (replaces
 // cufftExecZ2Z): one block per transform, ping-pong between two buffers.
 // ---------------------------------------------------------------------------
-__global__ void k_spectrum_prep(Dev d, int64_t T, double dt, double sigma, int nfft, double2* buf)
+// wts[Bp] (slot order, may be null): C(n) = sum_b wts[b] C_b(n) instead of
+// the batch mean
+__global__ void k_spectrum_prep(Dev d, int64_t T, double dt, double sigma, int nfft, double2* buf,
+                                const double* wts)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nfft; k += gridDim.x * blockDim.x) {
         double2 v = make_double2(0.0, 0.0);
         if (k <= T) {
             double sr = 0.0, si = 0.0;
-            for (int b = 0; b < d.B; ++b) {
+            for (int b = 0; b < d.Bp; ++b) {
                 const double2 c = d.corr[(size_t)k * d.Bp + b];
-                sr += c.x;
-                si += c.y;
+                if (wts) {
+                    sr += wts[b] * c.x;
+                    si += wts[b] * c.y;
+                } else if (b < d.B) {
+                    sr += c.x;
+                    si += c.y;
+                }
             }
-            double w = 1.0 / (double)d.B;
+            double w = wts ? 1.0 : 1.0 / (double)d.B;
             if (sigma > 0.0 && k < T) {
                 const double t = dt * (double)(k + 1);
                 w *= exp(-0.5 * (t / sigma) * (t / sigma));
@@ -1302,9 +1312,9 @@ __global__ void k_spectrum_prep(Dev d, int64_t T, double dt, double sigma, int n
 }
 
 hipError_t launch_spectrum_prep(const Dev& d, int64_t T, double dt, double sigma, int nfft,
-                                double2* buf, hipStream_t s)
+                                double2* buf, const double* wts, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_spectrum_prep, dim3((nfft + 255) / 256), dim3(256), 0, s, d, T, dt, sigma, nfft, buf);
+    hipLaunchKernelGGL(k_spectrum_prep, dim3((nfft + 255) / 256), dim3(256), 0, s, d, T, dt, sigma, nfft, buf, wts);
     return hipGetLastError();
 }
 

@@ -149,6 +149,20 @@ dq_status dq_system_reset_disorder(dq_system *sys);
  * are ignored (systemClass.cu:1325-1508) */
 dq_status dq_system_set_state(dq_system *sys, const double *psi);
 dq_status dq_system_get_state(dq_system *sys, double *psi);
+/* Resume point of a run: the state as the next step will use it (a
+ * normalisation still pending from dq_system_set_state is applied first),
+ * the OU offsets noise[b][i] (cm^-1), the error sums err_sum[b] (may be NULL)
+ * and the step counter.  dq_system_restore puts them back, bit for bit, into
+ * a system set up the same way (Hamiltonian, noise parameters, trajectories,
+ * dq_system_reset_disorder): the static disorder and the noise of later
+ * steps follow from the Philox counters, so a run split at any step and
+ * continued in another process equals the unsplit run bit for bit.  Records of
+ * observables before the restored step are not part of the snapshot.  No
+ * counterpart in the reference (its runs restart from the initial state);
+ * this is the checkpoint/resume of a long ensemble run. */
+dq_status dq_system_snapshot(dq_system *sys, double *psi, double *noise, double *err_sum, int64_t *step);
+dq_status dq_system_restore(dq_system *sys, const double *psi, const double *noise, const double *err_sum,
+                            int64_t step);
 /* Observables: site i adds |psi_i|^2 to group[i] (< 0: none), recorded
  * every `every` steps from step 0 up to max_steps; when ngroups == N and
  * group[i] == i the per-site populations are stored directly
@@ -156,6 +170,14 @@ dq_status dq_system_get_state(dq_system *sys, double *psi);
  * enables C(n) = <bra|psi(n)> every step (spectraClass.cu:509-571). */
 dq_status dq_system_set_observables(dq_system *sys, const int32_t *group, int32_t ngroups,
                                     int32_t every, int64_t max_steps, const double *bra);
+/* Several bras: C_b(n) = <bras[bra_of_traj[b]]|psi_b(n)>, bras[nbra][n]
+ * interleaved complex, bra_of_traj[b] < nbra per caller trajectory.  The
+ * reference runs one ensemble per dipole direction d = x, y, z, each with
+ * psi(0) = bra = mu_d / |mu_d| (spectraClass.cu:437-461), and averages the
+ * three spectra (spectraClass.cu:699-744); here the three directions run as
+ * one batch and dq_spectrum_weighted forms the average.  Call after
+ * dq_system_set_observables (which returns to its single bra). */
+dq_status dq_system_set_bras(dq_system *sys, int32_t nbra, const double *bras, const int32_t *bra_of_traj);
 /* Participation ratio PR_b(n) = 1 / sum_i |psi_b,i(n)|^4 of the normalised
  * state, every step n = 0..max_steps of the current observables
  * (systemClass.cu:2209-2257, called before each step at propagate.cu:208;
@@ -238,6 +260,13 @@ dq_status dq_pade_tbar(int32_t m, int32_t count, const double *alpha, const doub
  * spec[nfft] interleaved complex. */
 dq_status dq_spectrum(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
                       double *omega, double *spec);
+/* dq_spectrum of C(n) = sum_b weight[b] C_b(n) (weights per caller
+ * trajectory) and scale 1/T: with weight[b] = |mu_d(b)|^2 / (3 B_d) over three
+ * direction ensembles of B_d trajectories each it is the reference's
+ * orientational average of the absorption spectrum (spectraClass.cu:699-744;
+ * dq_spectrum is the case weight[b] = mu2 / B). */
+dq_status dq_spectrum_weighted(dq_system *sys, double sigma_fs, const double *weight, int32_t nfft,
+                               double *omega, double *spec);
 /* In-place FFTs of `batch` interleaved complex rows of length n (power of
  * two, <= 65536) on the GPU (replaces cufftExecZ2Z, spectraClass.cu:617). */
 dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
@@ -295,7 +324,16 @@ typedef struct dq_tuning {
     int32_t concurrent_slices;   /* 1, 2 or 4: a graph-captured step runs that many equal slices
                                     of the trajectory slots on their own streams (each slice a
                                     multiple of 64 slots) */
-    int32_t reserved[2];
+    int32_t spmm_window_rows;    /* 0: the row-blocked SpMMs above.  T > 0 (a multiple of 8):
+                                    the LDS-window SpMM (mixed precision only): a workgroup
+                                    stages the x rows a tile of T rows references in LDS and
+                                    multiplies 4-row blocks on the matrix cores; needs
+                                    spmm_rows_per_wave = 4, spmm_phases = 1, spmm_pipeline = 0,
+                                    spmm_traj_per_lane 1 or 2 (16 or 32 trajectories per wave)
+                                    and every tile's window (8 * 16 * spmm_traj_per_lane bytes
+                                    per referenced row, a banded matrix keeps it short) within
+                                    160 KiB, else DQ_ERR_INVALID_ARG */
+    int32_t spmm_window_waves;   /* 4, 8 or 16 waves per workgroup of the LDS-window SpMM */
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

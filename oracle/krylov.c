@@ -583,11 +583,20 @@ int orc_run(const orc_params *p, const orc_site_params *sp,
     double *de = e0 + n, *diag = de + n;
     double *work = (double *)malloc(sizeof(double) * (2 * (m + 2) + nt * nt * 5));
     orc_lanczos_opts lo = {m, p->reorth, p->breakdown_tol};
-    memcpy(psi, psi0, cs * sizeof(double));
-    orc_init_disorder(p, sp, traj, e0, de);
-    double tau = p->dt_fs / (double)p->krylov_substeps;
     double esum = 0.0;
     int rc = 0;
+    memcpy(psi, psi0, cs * sizeof(double));
+    /* the initial state is normalised (systemClass.cu:1383-1481,
+     * normalizeHostCol): psi / sqrt(sum |psi_i|^2), summed site by site */
+    {
+        double s2 = 0.0;
+        for (int i = 0; i < n; ++i) s2 += psi[2 * i] * psi[2 * i] + psi[2 * i + 1] * psi[2 * i + 1];
+        if (!(s2 > 0.0)) { rc = -2; goto done; }
+        double inv = 1.0 / sqrt(s2);
+        for (size_t i = 0; i < cs; ++i) psi[i] *= inv;
+    }
+    orc_init_disorder(p, sp, traj, e0, de);
+    double tau = p->dt_fs / (double)p->krylov_substeps;
     if (pops && obs_every > 0) record(n, psi, group, ngroups, pops);
     if (corr && bra) cdot(n, bra, psi, &corr[0], &corr[1]);
     if (partic) partic[0] = orc_participation(n, psi);

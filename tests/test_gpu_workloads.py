@@ -5,9 +5,7 @@ the FP64 oracle by test_gpu_run.py and test_gpu_tuning.py).
 * ragged batch: B = 256 double-walled tubes with ring counts uniform in
   [20, 200] per wall (N_b = 120 r_b, 2,400 .. 24,000 sites), m = 24,
   dt = 2 fs, 2000 steps, autotuned launch choices;
-* spectra: B = 512 realisations of the N = 12,000 tube, dt = 1 fs, 4096
-  steps, C(t) = <mu_x|psi(t)> accumulated in FP64, Gaussian window 1 ps,
-  zero-padded FFT of length 8192."""
+* spectra (the three-direction average, B = 3 x 512): test_gpu_isotropic.py."""
 import numpy as np
 import pytest
 
@@ -72,55 +70,3 @@ def test_ragged_batch_mixed_vs_fp64(engine):
     assert err <= 1e-6
     for b in range(256):
         assert np.all(sm[b, na[b]:] == 0)
-
-
-def test_spectrum_b512_mixed_vs_fp64(engine):
-    sysd = double_tube(engine)
-    n = sysd["n"]
-    mu = sysd["dip"][:, 0]
-    mu2 = float(mu @ mu)
-    bra = (mu / np.sqrt(mu2)).astype(complex)
-    steps, nfft, sigma = 4096, 8192, 1000.0
-
-    def mk(B, prec):
-        s = engine.System(n, 24, 1.0, n_traj=B, precision=prec, sigma_static=250.0, seed=7)
-        s.set_hamiltonian(sysd["rp"], sysd["col"], sysd["val"], np.zeros(n))
-        s.set_ou(50.0, 400.0)
-        s.reset_disorder()
-        s.set_state(bra)
-        s.set_observables(None, 0, 0, steps, bra=bra)
-        return s
-
-    s = mk(512, engine.MIXED)
-    s.run(steps, graph=True)
-    ms = s.last_run_ms()
-    om, spec512 = s.spectrum(sigma, mu2, nfft)
-    cm = s.corr()
-    s.close()
-    f = mk(64, engine.FP64)
-    f.run(steps, graph=True)
-    om64, spec64 = f.spectrum(sigma, mu2, nfft)
-    cf = f.corr()
-    f.close()
-    assert np.array_equal(om, om64)
-
-    def spectrum_of(c):
-        """the engine's spectrum formula in numpy, for a subset average"""
-        k = np.arange(steps + 1)
-        w = np.where(k < steps, np.exp(-0.5 * ((k + 1) * 1.0 / sigma) ** 2), 1.0)
-        x = np.zeros(nfft, complex)
-        x[:steps + 1] = c * w
-        return np.fft.ifft(x) * nfft * mu2 / steps
-
-    # the GPU FFT path at full size against numpy, on the FP64 average
-    assert np.abs(spec64 - spectrum_of(cf.mean(0))).max() <= 1e-12 * np.abs(spec64).max()
-    a, b = spectrum_of(cm[:64].mean(0)), spectrum_of(cf.mean(0))
-    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-    corr_err = np.abs(cm[:64] - cf).max()
-    print("spectra B=512 N=12000 4096 steps: %.3f ms/step, %.0f traj-steps/s"
-          % (ms / steps, 512 * steps / (ms / 1e3)))
-    print("mixed vs FP64 on 64 realisations: spectrum relative L2 error %.3e, max |C_mixed - C_fp64| %.3e"
-          % (rel, corr_err))
-    peak = om[np.argmax(spec512.real)]
-    print("B=512 spectrum peak at %.1f cm^-1" % peak)
-    assert rel <= 1e-4
