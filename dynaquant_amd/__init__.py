@@ -33,7 +33,7 @@ EXPORTS = [
     "dq_read_spectral_density", "dq_parse_infile", "dq_write_pops_file", "dq_read_pops_file",
     "dq_read_position_file", "dq_cd_vectors", "dq_system_set_participation", "dq_get_participation",
     "dq_system_set_taylor_krylov", "dq_system_snapshot", "dq_system_restore", "dq_system_set_bras",
-    "dq_spectrum_weighted",
+    "dq_spectrum_weighted", "dq_system_set_krylov_control", "dq_get_krylov_control",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -236,6 +236,17 @@ class System:
         a = np.zeros(2 * self.n * self.B)
         _check(lib().dq_system_get_state(self._h, _p(a)))
         return a.view(np.complex128).reshape(self.B, self.n)
+
+    def set_krylov_control(self, tol, k_max=16, chunk=10):
+        """substeps from the Expokit estimate: tol per dt (<= 0: off)"""
+        _check(lib().dq_system_set_krylov_control(self._h, C.c_double(tol), int(k_max), int(chunk)))
+
+    def krylov_control(self):
+        """(current K, chunks redone)"""
+        k = C.c_int32()
+        r = C.c_int64()
+        _check(lib().dq_get_krylov_control(self._h, C.byref(k), C.byref(r)))
+        return int(k.value), int(r.value)
 
     def snapshot(self):
         """resume point: dict(psi [B, n], noise [B, n], err_sum [B], step)"""

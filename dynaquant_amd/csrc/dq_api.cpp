@@ -84,16 +84,12 @@ static size_t win_lds_limit()
     return lim;
 }
 
-// Step lists of the LDS-window SpMM (dq_window.hip) for tiles of T rows and
-// slices of ts trajectories: per tile its window (first row, rows), per MFMA
-// group of G = 32 / ts four-row blocks the first step, and per batch of four
-// steps a record of 18 G dwords: the couplings [g][step][row], then the
-// uint16 window offsets [step][g], two per dword.
+// Windows of the LDS-window SpMM (dq_window.hip) for tiles of T rows: per
+// tile the first row and the number of rows of every column its rows
+// reference and the rows themselves; the products read the rb = 16 lists.
 struct WinLists {
-    int32_t* lo = nullptr;
-    int32_t* gptr = nullptr;
-    uint32_t* rec = nullptr;         // none for 16-row blocks (they read the rb = 16 lists)
-    int T = 0, ts = 0, R = 0, max_rows = 0, ntile = 0;
+    int32_t* lo = nullptr;           // [tile][2] first window row, window rows
+    int T = 0, ts = 0, max_rows = 0, ntile = 0;
 };
 
 struct dq_system {
@@ -106,6 +102,20 @@ struct dq_system {
     hipEvent_t ev_fork = nullptr, ev_join[kMaxSlices - 1] = {};
     int slices = 1;                   // captured steps run this many trajectory slices concurrently
     WinLists win;                     // LDS-window SpMM lists (win.T > 0 when built)
+    // substep control (dq_system_set_krylov_control): chunks of kchunk steps,
+    // K doubled up to kmax while K x the largest cycle estimate exceeds ktol
+    double hbar = 0.0;
+    double ktol = 0.0;
+    int kmax = 0, kchunk = 10, K0 = 1;
+    int64_t k_rejects = 0;
+    double* ck_err_max = nullptr;     // [Bp] device maxima, kept while control is off
+    struct {
+        double2* psi = nullptr;
+        double *de = nullptr, *err = nullptr, *nrm2 = nullptr;
+        void* diag = nullptr;
+        int* status = nullptr;
+        int64_t* step = nullptr;
+    } ck;
     int win_waves = 8;                // waves per workgroup of the window SpMM
     bool win_lds_set = false;         // dynamic LDS limit raised for the window SpMM
     hipGraphExec_t graph = nullptr;
@@ -311,7 +321,7 @@ static void refresh_spmm_tiles(dq_system* sys)
         d.rt_s = 4 * d.rb;
         d.ntile_s = (sys->n + d.rt_s - 1) / d.rt_s;
     }
-    d.ntile_p = (d.win_T > 0 && sys->win.lo) ? (sys->win.R == 4 ? d.win_strips : sys->win.ntile) : d.ntile_s;
+    d.ntile_p = (d.win_T > 0 && sys->win.lo) ? sys->win.ntile : d.ntile_s;
 }
 
 // row blocks of R rows for the Hamiltonian (from the host CSR kept by
@@ -345,111 +355,41 @@ static dq_status build_row_blocks(dq_system* sys, int R)
 static void release_window(dq_system* sys, WinLists* w)
 {
     sys->release(w->lo);
-    sys->release(w->gptr);
-    sys->release(w->rec);
     *w = WinLists{};
 }
 
-// Lists of the LDS-window SpMMs (dq_window.hip) for tiles of T rows.
-//  * 16-row blocks (R = 16, slices of ts = 16 or 32 trajectories): per tile the
-//    window [lo, lo + rows) of every column its rows reference and the rows
-//    themselves; the products read the rb = 16 row-block lists.
-//  * 4-row blocks (R = 4, slices of 32 trajectories, persistent strips with an
-//    LDS ring): per tile lo' (suffix minimum of the windows' first rows) and
-//    hi' (prefix maximum of their last rows), the ring size RING = max hi' -
-//    lo' + 1, and per 4-row block its union of columns (ascending, as in the
-//    4-row blocks of make_row_blocks) in batches of four steps, each a record
-//    of 16 couplings [step][row] and the four columns' ring slots (column mod
-//    RING, uint16, two per dword); padding steps repeat the block's first
-//    column with zero couplings.
-// Fails (nothing changed) when T does not suit the blocks, or a window or the
-// ring does not fit 65,536 rows or one workgroup's LDS.
-static dq_status make_window_lists(dq_system* sys, int T, int ts, int R, WinLists* out)
+// Windows of the LDS-window SpMM (dq_window.hip) for tiles of T rows (a
+// multiple of 16: whole 16-row blocks) and slices of ts = 16 or 32
+// trajectories.  Fails (nothing changed) when a window does not fit 65,536
+// rows or one workgroup's LDS.
+static dq_status make_window_lists(dq_system* sys, int T, int ts, WinLists* out)
 {
     const int n = sys->n;
-    if (R != 4 && R != 16) return fail(DQ_ERR_INVALID_ARG, "window SpMM: row blocks of 4 or 16 rows");
-    if (R == 16 ? (ts != 16 && ts != 32) : ts != 32)
-        return fail(DQ_ERR_INVALID_ARG, "window SpMM: 16 or 32 trajectories per slice (32 with 4-row blocks)");
-    if (T < 16 || T % (R == 16 ? 16 : 8) != 0 || T > 4096)
-        return fail(DQ_ERR_INVALID_ARG, "window SpMM: tile rows must be a multiple of 8 (16 for 16-row blocks) "
-                                        "in [16, 4096]");
+    if (ts != 16 && ts != 32) return fail(DQ_ERR_INVALID_ARG, "window SpMM: 16 or 32 trajectories per slice");
+    if (T < 16 || T % 16 != 0 || T > 4096)
+        return fail(DQ_ERR_INVALID_ARG, "window SpMM: tile rows must be a multiple of 16 in [16, 4096]");
     const int32_t* rowptr = sys->rowptr.data();
     const int32_t* col = sys->col.data();
-    const double* val = sys->val.data();
     const int ntile = (n + T - 1) / T;
-    std::vector<int32_t> lo(2 * (size_t)ntile), hi(ntile), gptr;
+    std::vector<int32_t> lo(2 * (size_t)ntile);
+    int max_rows = 0;
     for (int t = 0; t < ntile; ++t) {
         const int r0 = t * T, r1 = std::min(n, r0 + T);
         int mn = r0, mx = r1 - 1;
         for (int i = r0; i < r1; ++i)
             for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) { mn = std::min(mn, col[p]); mx = std::max(mx, col[p]); }
         lo[2 * t] = mn;
-        hi[t] = mx;
-    }
-    int max_rows = 0;
-    std::vector<uint32_t> rec;
-    constexpr size_t kRec = 18;
-    if (R == 16) {
-        for (int t = 0; t < ntile; ++t) {
-            lo[2 * t + 1] = hi[t] - lo[2 * t] + 1;
-            max_rows = std::max(max_rows, lo[2 * t + 1]);
-        }
-    } else {
-        // lo' and hi', the ring and the largest number of new rows per tile
-        for (int t = ntile - 2; t >= 0; --t) lo[2 * t] = std::min(lo[2 * t], lo[2 * t + 2]);
-        for (int t = 1; t < ntile; ++t) hi[t] = std::max(hi[t], hi[t - 1]);
-        for (int t = 0; t < ntile; ++t) {
-            lo[2 * t + 1] = hi[t];
-            max_rows = std::max(max_rows, hi[t] - lo[2 * t] + 1);
-        }
-        const int ring = max_rows;
-        const int nblk = (n + 3) / 4;
-        gptr.assign((size_t)nblk + 1, 0);
-        std::vector<int32_t> u;
-        size_t steps = 0;
-        for (int rb = 0; rb < nblk; ++rb) {
-            u.clear();
-            for (int i = 4 * rb; i < std::min(n, 4 * rb + 4); ++i) u.insert(u.end(), col + rowptr[i], col + rowptr[i + 1]);
-            std::sort(u.begin(), u.end());
-            u.erase(std::unique(u.begin(), u.end()), u.end());
-            const size_t L = (u.size() + 3) / 4 * 4;
-            rec.resize((steps + L) / 4 * kRec, 0u);
-            for (size_t k = 0; k < L; ++k) {
-                const size_t st = steps + k, b = st / 4, kk = st % 4;
-                const uint32_t slot = (uint32_t)((k < u.size() ? u[k] : u[0]) % ring);
-                rec[b * kRec + 16 + kk / 2] |= slot << (16 * (kk & 1));
-            }
-            for (int i = 4 * rb; i < std::min(n, 4 * rb + 4); ++i)
-                for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-                    const size_t pos = std::lower_bound(u.begin(), u.end(), col[p]) - u.begin();
-                    const size_t st = steps + pos, b = st / 4, kk = st % 4;
-                    const float v = (float)val[p];
-                    uint32_t bits;
-                    std::memcpy(&bits, &v, 4);
-                    rec[b * kRec + kk * 4 + (size_t)(i - 4 * rb)] = bits;
-                }
-            steps += L;
-            if (steps / 4 * kRec * 4 > (size_t)INT32_MAX) return fail(DQ_ERR_INVALID_ARG, "window SpMM: too many steps");
-            gptr[rb + 1] = (int32_t)steps;
-        }
+        lo[2 * t + 1] = mx - mn + 1;
+        max_rows = std::max(max_rows, mx - mn + 1);
     }
     if (max_rows > 65536) return fail(DQ_ERR_INVALID_ARG, "window SpMM: a window exceeds 65,536 rows");
     const size_t lds = dq::spmm_win_lds_bytes(ts, max_rows), lds_max = win_lds_limit();
     if (lds > lds_max)
         return fail(DQ_ERR_INVALID_ARG, "window SpMM: the widest window needs " + std::to_string(lds) +
                                             " bytes of LDS, more than one workgroup's " + std::to_string(lds_max));
-    rec.resize(std::max<size_t>(rec.size(), kRec), 0u);
-    gptr.resize(std::max<size_t>(gptr.size(), 1), 0);
     WinLists w;
-    dq_status st = DQ_OK;
-    if ((st = sys->alloc(&w.lo, lo.size())) != DQ_OK || (st = sys->alloc(&w.gptr, gptr.size())) != DQ_OK ||
-        (st = sys->alloc(&w.rec, rec.size())) != DQ_OK) {
-        release_window(sys, &w);
-        return st;
-    }
+    DQ_TRY(sys->alloc(&w.lo, lo.size()));
     hipError_t e = hipMemcpyAsync(w.lo, lo.data(), sizeof(int32_t) * lo.size(), hipMemcpyHostToDevice, sys->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(w.gptr, gptr.data(), sizeof(int32_t) * gptr.size(), hipMemcpyHostToDevice, sys->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(w.rec, rec.data(), sizeof(uint32_t) * rec.size(), hipMemcpyHostToDevice, sys->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
     if (e != hipSuccess) {
         release_window(sys, &w);
@@ -467,7 +407,6 @@ static dq_status make_window_lists(dq_system* sys, int T, int ts, int R, WinList
     }
     w.T = T;
     w.ts = ts;
-    w.R = R;
     w.max_rows = max_rows;
     w.ntile = ntile;
     release_window(sys, out);
@@ -486,21 +425,6 @@ static void attach_window(dq_system* sys)
     d.win_w = sys->win_waves;
     d.win_lds = d.win_T > 0 ? (int)dq::spmm_win_lds_bytes(w.ts, w.max_rows) : 0;
     d.win_lo = w.lo;
-    d.win_gptr = w.gptr;
-    d.win_rec = w.rec;
-    d.win_ring = w.R == 4 ? w.max_rows : 0;
-    d.win_strips = d.win_tps = 0;
-    if (d.win_T > 0 && w.R == 4) {
-        // one persistent workgroup per CU over the whole batch: strips of
-        // consecutive tiles x slices of 32 trajectories
-        int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            cus = 256;
-        const int nsl = std::max(1, d.Bp / 32);
-        const int want = std::max(1, (cus + nsl - 1) / nsl);
-        d.win_tps = (w.ntile + want - 1) / want;
-        d.win_strips = (w.ntile + d.win_tps - 1) / d.win_tps;
-    }
     refresh_spmm_tiles(sys);
 }
 
@@ -568,6 +492,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.sqrt_subdt = std::sqrt(d.subdt);
     d.g = cfg->coupling_scale;
     const double hbar = cfg->hbar_cm_fs > 0.0 ? cfg->hbar_cm_fs : reference_hbar();
+    s->hbar = hbar;
+    s->K0 = cfg->krylov_substeps;
     // propagate.cu:90-92: expCoeff = -i (1/hbar) dt, here per Krylov substep
     d.theta = -((-1. / hbar) * (cfg->dt_fs / (double)cfg->krylov_substeps));
     d.tol = cfg->breakdown_tol > 0.0 ? cfg->breakdown_tol : (d.mixed ? 1e-5 : 1e-10);
@@ -702,7 +628,7 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
     };
     WinLists nw;
     if (sys->win.T > 0) {
-        dq_status st = make_window_lists(sys, sys->win.T, sys->win.ts, sys->win.R, &nw);
+        dq_status st = make_window_lists(sys, sys->win.T, sys->win.ts, &nw);
         if (st != DQ_OK) { restore_host(); return st; }
     }
     dq_status st = build_row_blocks(sys, d.rb);
@@ -756,6 +682,7 @@ dq_status dq_system_set_taylor_krylov(dq_system* sys, int32_t p, int32_t m_aux, 
     if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
     Dev& d = sys->d;
     const int n = sys->n;
+    if (p != 0 && sys->ktol > 0.0) return fail(DQ_ERR_INVALID_ARG, "substep control is for the Lanczos propagator");
     if (p != 0) {
         if (p < 2 || m_aux < 2 || m_aux > sys->m || m_aux + p > dq::kMaxM + 2)
             return fail(DQ_ERR_INVALID_ARG, "Taylor-Krylov needs p >= 2, 2 <= m_aux <= m and m_aux + p <= 34");
@@ -1401,14 +1328,13 @@ dq_status dq_propagate_step(dq_system* sys)
     return dq_run(sys, 1, 0);
 }
 
-dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
+}  // extern "C"
+
+// nsteps steps enqueued (the graph captured first when asked for), timed by
+// events; no finalisation
+static dq_status run_steps(dq_system* sys, int64_t nsteps, int use_graph, float* ms_out)
 {
-    DQ_TRY(check_ready(sys));
-    if (nsteps < 0) return fail(DQ_ERR_INVALID_ARG, "nsteps < 0");
-    if (dq::obs_on(sys->d) && sys->step + nsteps > sys->max_steps)
-        return fail(DQ_ERR_INVALID_ARG, "run exceeds max_steps of the observables");
     hipStream_t st = sys->stream;
-    if (sys->profiling) use_graph = 0;
     if (use_graph && nsteps > 0 && !sys->graph) {
         hipGraph_t g;
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1427,14 +1353,156 @@ dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
         if (sys->profiling && sys->ev_used > 4096) HIP_TRY(collect_timings(sys));
     }
     HIP_TRY(hipEventRecord(sys->ev1, st));
-    HIP_TRY(enqueue_finalize(sys));
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(collect_timings(sys));
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+    *ms_out = ms;
+    return DQ_OK;
+}
+
+// K Krylov cycles per dt from now on: tau = dt / K
+static void set_substeps(dq_system* sys, int K)
+{
+    if (K == sys->K) return;
+    sys->drop_graph();
+    sys->K = K;
+    // propagate.cu:90-92, per Krylov substep, the expression of dq_system_create
+    sys->d.theta = -((-1. / sys->hbar) * (sys->cfg.dt_fs / (double)K));
+}
+
+// the state a chunk starts from, for a redo with more substeps
+static dq_status chunk_save(dq_system* sys, bool restore)
+{
+    Dev& d = sys->d;
+    hipStream_t st = sys->stream;
+    const size_t nB = (size_t)sys->n * d.Bp, vb = d.mixed ? sizeof(float) : sizeof(double);
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+    };
+    auto& c = sys->ck;
+    if (!restore) {
+        HIP_TRY(cp(c.psi, d.psi, sizeof(double2) * nB));
+        HIP_TRY(cp(c.de, d.de, sizeof(double) * nB));
+        HIP_TRY(cp(c.diag, d.diag, vb * nB));
+        HIP_TRY(cp(c.err, d.err_sum, sizeof(double) * d.Bp));
+        HIP_TRY(cp(c.nrm2, d.nrm2, sizeof(double) * d.Bp));
+        HIP_TRY(cp(c.status, d.status, sizeof(int)));
+        HIP_TRY(cp(c.step, d.step, sizeof(int64_t)));
+    } else {
+        HIP_TRY(cp(d.psi, c.psi, sizeof(double2) * nB));
+        HIP_TRY(cp(d.de, c.de, sizeof(double) * nB));
+        HIP_TRY(cp(d.diag, c.diag, vb * nB));
+        HIP_TRY(cp(d.err_sum, c.err, sizeof(double) * d.Bp));
+        HIP_TRY(cp(d.nrm2, c.nrm2, sizeof(double) * d.Bp));
+        HIP_TRY(cp(d.status, c.status, sizeof(int)));
+        HIP_TRY(cp(d.step, c.step, sizeof(int64_t)));
+    }
+    return DQ_OK;
+}
+
+// Substep control from the Expokit estimate (padeClass.cu:749-782,
+// systemClass.cu:2055-2085): the reference sums the estimate; here a chunk
+// of steps is accepted when K times the largest clamped estimate of any
+// cycle of any trajectory stays within the tolerance per step, and redone
+// from its start with twice the substeps otherwise (up to kmax).
+static dq_status run_controlled(dq_system* sys, int64_t nsteps, int use_graph, float* ms_total)
+{
+    Dev& d = sys->d;
+    hipStream_t st = sys->stream;
+    std::vector<double> em(d.Bp);
+    int64_t done = 0;
+    float total = 0.f;
+    while (done < nsteps) {
+        const int64_t c = std::min<int64_t>(sys->kchunk, nsteps - done);
+        DQ_TRY(chunk_save(sys, false));
+        HIP_TRY(hipMemsetAsync(d.err_max, 0, sizeof(double) * d.Bp, st));
+        float ms = 0.f;
+        DQ_TRY(run_steps(sys, c, use_graph, &ms));
+        total += ms;
+        HIP_TRY(hipMemcpy(em.data(), d.err_max, sizeof(double) * d.Bp, hipMemcpyDeviceToHost));
+        double emax = 0.0;
+        for (int b = 0; b < sys->B; ++b) emax = std::max(emax, em[sys->slot[b]]);
+        if (!(emax * sys->K <= sys->ktol) && 2 * sys->K <= sys->kmax) {
+            DQ_TRY(chunk_save(sys, true));
+            set_substeps(sys, 2 * sys->K);
+            ++sys->k_rejects;
+            continue;
+        }
+        done += c;
+        sys->step += c;
+    }
+    sys->step -= nsteps;               // dq_run adds the whole run
+    *ms_total = total;
+    return DQ_OK;
+}
+
+extern "C" {
+
+dq_status dq_run(dq_system* sys, int64_t nsteps, int32_t use_graph)
+{
+    DQ_TRY(check_ready(sys));
+    if (nsteps < 0) return fail(DQ_ERR_INVALID_ARG, "nsteps < 0");
+    if (dq::obs_on(sys->d) && sys->step + nsteps > sys->max_steps)
+        return fail(DQ_ERR_INVALID_ARG, "run exceeds max_steps of the observables");
+    if (sys->profiling) use_graph = 0;
+    float ms = 0.f;
+    if (sys->ktol > 0.0) DQ_TRY(run_controlled(sys, nsteps, use_graph, &ms));
+    else DQ_TRY(run_steps(sys, nsteps, use_graph, &ms));
+    HIP_TRY(enqueue_finalize(sys));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
     sys->last_ms = ms;
     sys->step += nsteps;
     return dq_sync(sys);
+}
+
+dq_status dq_system_set_krylov_control(dq_system* sys, double tol, int32_t k_max, int32_t chunk_steps)
+{
+    if (!sys) return fail(DQ_ERR_INVALID_ARG, "null system");
+    Dev& d = sys->d;
+    if (!(tol > 0.0)) {
+        sys->ktol = 0.0;
+        d.err_max = nullptr;
+        sys->drop_graph();
+        set_substeps(sys, sys->K0);
+        return DQ_OK;
+    }
+    if (sys->tk.p > 0) return fail(DQ_ERR_INVALID_ARG, "substep control is for the Lanczos propagator");
+    if (k_max < sys->K0 || chunk_steps < 1) return fail(DQ_ERR_INVALID_ARG, "need k_max >= krylov_substeps and chunk_steps >= 1");
+    sys->drop_graph();
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    auto& c = sys->ck;
+    if (!c.psi) {
+        const size_t nB = (size_t)sys->n * d.Bp;
+        Pending pend(sys);
+        double* em;
+        DQ_TRY(pend.alloc(&em, (size_t)d.Bp));
+        DQ_TRY(pend.alloc(&c.psi, nB));
+        DQ_TRY(pend.alloc(&c.de, nB));
+        if (d.mixed) { float* f; DQ_TRY(pend.alloc(&f, nB)); c.diag = f; }
+        else { double* f; DQ_TRY(pend.alloc(&f, nB)); c.diag = f; }
+        DQ_TRY(pend.alloc(&c.err, (size_t)d.Bp));
+        DQ_TRY(pend.alloc(&c.nrm2, (size_t)d.Bp));
+        DQ_TRY(pend.alloc(&c.status, 1));
+        DQ_TRY(pend.alloc(&c.step, 1));
+        pend.keep();
+        d.err_max = em;
+        sys->ck_err_max = em;
+    } else {
+        d.err_max = sys->ck_err_max;
+    }
+    sys->ktol = tol;
+    sys->kmax = k_max;
+    sys->kchunk = chunk_steps;
+    return DQ_OK;
+}
+
+dq_status dq_get_krylov_control(dq_system* sys, int32_t* k_now, int64_t* rejected_chunks)
+{
+    if (!sys || !k_now || !rejected_chunks) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    *k_now = sys->K;
+    *rejected_chunks = sys->k_rejects;
+    return DQ_OK;
 }
 
 dq_status dq_set_profiling(dq_system* sys, int32_t on)
@@ -2035,18 +2103,14 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
             *why = "the LDS-window SpMM (spmm_window_rows > 0) is for mixed precision";
             return false;
         }
-        if ((t.spmm_rows_per_wave != 4 && t.spmm_rows_per_wave != 16) ||
-            (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) ||
-            (t.spmm_rows_per_wave == 4 && t.spmm_traj_per_lane != 2) || t.spmm_phases != 1 ||
-            t.spmm_pipeline != 0) {
-            *why = "the LDS-window SpMM needs spmm_rows_per_wave 16 with spmm_traj_per_lane 1 or 2 (16 or 32 "
-                   "trajectories per slice) or 4 with spmm_traj_per_lane 2, spmm_phases = 1 and "
-                   "spmm_pipeline = 0";
+        if (t.spmm_rows_per_wave != 16 || (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) ||
+            t.spmm_phases != 1 || t.spmm_pipeline != 0) {
+            *why = "the LDS-window SpMM needs spmm_rows_per_wave = 16, spmm_traj_per_lane 1 or 2 (16 or 32 "
+                   "trajectories per slice), spmm_phases = 1 and spmm_pipeline = 0";
             return false;
         }
-        const int mult = t.spmm_rows_per_wave == 16 ? 16 : 8;
-        if (t.spmm_window_rows % mult != 0 || t.spmm_window_rows < 16 || t.spmm_window_rows > 4096) {
-            *why = "spmm_window_rows must be a multiple of 8 (16 with 16-row blocks) in [16, 4096]";
+        if (t.spmm_window_rows % 16 != 0 || t.spmm_window_rows > 4096) {
+            *why = "spmm_window_rows must be a multiple of 16, at most 4096";
             return false;
         }
         if (t.spmm_window_waves != 0 && t.spmm_window_waves != 4 && t.spmm_window_waves != 8 &&
@@ -2079,10 +2143,8 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     Dev& d = sys->d;
     const int T = t.spmm_window_rows, ts = 16 * t.spmm_traj_per_lane;
     WinLists nw;
-    const int R = t.spmm_rows_per_wave;
-    const bool new_win =
-        T > 0 && sys->have_h && (sys->win.T != T || sys->win.ts != ts || sys->win.R != R || !sys->win.lo);
-    if (new_win) DQ_TRY(make_window_lists(sys, T, ts, R, &nw));
+    const bool new_win = T > 0 && sys->have_h && (sys->win.T != T || sys->win.ts != ts || !sys->win.lo);
+    if (new_win) DQ_TRY(make_window_lists(sys, T, ts, &nw));
     if (t.spmm_rows_per_wave != d.rb) {
         if (sys->have_h) {
             dq_status st = build_row_blocks(sys, t.spmm_rows_per_wave);
@@ -2104,7 +2166,6 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
         release_window(sys, &sys->win);
         sys->win.T = T;
         sys->win.ts = ts;
-        sys->win.R = R;
     }
     sys->win_waves = t.spmm_window_waves > 0 ? t.spmm_window_waves : 8;
     d.spmm_tpl = t.spmm_traj_per_lane;
@@ -2285,15 +2346,13 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                     cands.push_back(t);
                 }
     if (d.mixed)
-        for (int R : {16, 4})
-        for (int T : {240, 120, 64, 128, 256})
+        for (int T : {240, 128, 64, 256})
             for (int tpl : {2, 1})
                 for (int W : {8, 4, 16})
                     for (int cand = 0; cand < 4; ++cand) {
-                        if ((R == 16 && T % 16) || (R == 4 && tpl != 2)) continue;
                         dq_tuning t;
                         std::memset(&t, 0, sizeof(t));
-                        t.spmm_rows_per_wave = R;
+                        t.spmm_rows_per_wave = 16;
                         t.spmm_traj_per_lane = tpl;
                         t.spmm_phases = 1;
                         t.tile_rows = rows[cand];

@@ -16,7 +16,6 @@ constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
 constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
 constexpr int kRowsMin = 32;       // smallest elementwise tile (rows per workgroup)
-constexpr int kRollPre = 8;        // ring SpMM: float4 per thread held for the next tile's rows
 
 struct Dev {
     int n, Bp, B, m;
@@ -57,11 +56,7 @@ struct Dev {
     // LDS-window SpMM (dq_window.hip), mixed precision: tiles of win_T rows,
     // slices of win_ts trajectories, win_w waves per workgroup
     int win_T, win_ts, win_w, win_lds;
-    int win_ring, win_strips, win_tps;  // 4-row blocks: LDS ring rows, strips, tiles per strip
-    const int32_t* win_lo;          // [tile][2] 16-row blocks: first window row, window rows;
-                                    // 4-row blocks: lo' (suffix minimum), hi' (prefix maximum)
-    const int32_t* win_gptr;        // [row block + 1] first step of each 4-row block
-    const uint32_t* win_rec;        // [step / 4][18] batch records: couplings, uint16 ring slots
+    const int32_t* win_lo;          // [tile][2] first window row, window rows
     const double* inv_tau;          // [n]
     const double* coeff;            // [n] sigma*sqrt(2/tau)
     const double* ou_sigma;         // [n]
@@ -88,6 +83,7 @@ struct Dev {
     double* nrm2;                   // [Bp] |psi|^2 after combine
     double2* ycoef;                 // [m][Bp] y_k * s_k
     double* err_sum;                // [Bp]
+    double* err_max;                // [Bp] largest clamped estimate of a cycle (substep control), or null
     int* broke;                     // [Bp]
     int* status;                    // [1] sticky error flags
     double* part;                   // partial sums

@@ -938,6 +938,7 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
         } else {
             if (err < 1.110223e-16) err = 1.110223e-16;
             d.err_sum[b] += err;
+            if (d.err_max) d.err_max[b] = __builtin_fmax(d.err_max[b], err);
         }
     }
 }
@@ -1058,6 +1059,7 @@ __global__ __launch_bounds__(kThreads) void k_finish_norm(Dev d, int advance_ste
             double err = d.tk_err[b] + __builtin_fabs(1. - __builtin_sqrt(s));
             if (err < 1.110223e-16) err = 1.110223e-16;
             d.err_sum[b] += err;
+            if (d.err_max) d.err_max[b] = __builtin_fmax(d.err_max[b], err);
         }
     }
     if (advance_step && blockIdx.x == 0 && threadIdx.x == 0) *d.step = *d.step + 1;

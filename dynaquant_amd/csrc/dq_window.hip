@@ -3,31 +3,10 @@
 //
 // The Hamiltonian of an aggregate is banded in its site order (a dipole tube
 // in ring-major order couples a site only to rings r-3..r+3), so the x rows a
-// tile of T consecutive rows references form one short contiguous window.
-//
-// k_spmm_roll: a persistent workgroup owns a slice of 32 trajectories and a
-// strip of consecutive tiles.  It keeps the x rows of the current tile's
-// window in an LDS ring (row r in slot r mod RING, 256 bytes per row), loads
-// the next tile's new rows into registers while the current tile multiplies
-// and stores them after it: every x row is read from memory about once per
-// strip, and the loads never stall the matrix cores.  The product runs as
-// 4 x 4 x 1 outer products (v_mfma_f32_4x4x1_16b_f32, sixteen 4 x 4 blocks per
-// instruction): a row block is 4 consecutive rows over the sorted union of
-// their columns (1.36x the multiply-adds of plain CSR on the 12,000-site tube,
-// against 2.7x for 16-row blocks); one union column is one K = 1 step with
-// A = the 4 couplings of the block's rows and B = x of the slice's 64 real
-// columns (re/im of 32 trajectories).  Every output element is the sequence
-// c = fma(J_ik, x_k, c) over the union columns in ascending order, the same
-// sequence the 4-row VALU SpMM (dq_batched.hip, k_spmm_b with kRB = 4)
-// evaluates, so both give bitwise the same product.
-//
-// Step lists (built on the host, dq_api.cpp): the union columns of a row
-// block are walked in batches of four steps (padded with zero couplings at
-// slot 0).  A batch is one record of 18 dwords: the couplings [step][row]
-// (16 floats), then the ring slots of the four columns as uint16, two per
-// dword.  Each lane loads one dword: lanes 4 k + i hold the coupling of row i
-// at step k, which the matrix core broadcasts to all 16 blocks (CBSZ = 4,
-// ABID = k); lanes 16, 17 hold the slots, read back as wave-uniform values.
+// tile of T consecutive rows references form one short contiguous window
+// [lo, lo + rows).  A workgroup owns a tile and a slice of trajectories, copies
+// the window into LDS with coalesced 16-byte loads, and then reads every x row
+// of its products from LDS instead of gathering it from L2.
 #include "dq_device.h"
 #include "dq_internal.h"
 
@@ -35,181 +14,7 @@ namespace dq {
 
 static inline int wpad8(int x) { return (x + 7) & ~7; }
 
-constexpr int kDepth = 4;              // batches of four steps in flight per wave
-constexpr int kPre = kRollPre;         // float4 per thread held for the next tile's rows
-
-// ring slot of step k of a batch (record `r` as loaded by the wave)
-__device__ __forceinline__ int roll_slot(uint32_t r, int k)
-{
-    const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)r, 16 + (k >> 1));
-    return (int)((k & 1) ? (w >> 16) : (w & 0xffffu));
-}
-
-__global__ __launch_bounds__(1024) void k_spmm_roll(Dev d, int j, int last)
-{
-    constexpr int TS = 32, RW = 64, C4 = 16;   // trajectories, floats, float4 per ring row
-    typedef float mf4 __attribute__((ext_vector_type(4)));
-    extern __shared__ float ring[];
-    const int nsl = d.sn / TS;
-    const int logical = xcd_tile(blockIdx.x, gridDim.x);
-    if (logical >= d.win_strips * nsl) return;
-    // strip-major: the slices of a strip (sharing its step lists) run on one XCD
-    const int strip = logical / nsl;
-    const int b0 = d.s0 + (logical % nsl) * TS;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
-    const int nthr = blockDim.x;
-    const int Bp = d.Bp, T = d.win_T, RING = d.win_ring;
-    const int ntile = (d.n + T - 1) / T, nblk = (d.n + 3) >> 2;
-    // slots are ordered by n_active (largest first): the slice's first slot
-    // bounds the rows any of its trajectories uses
-    const int slice_nact = d.nact[b0];
-    const int t0 = strip * d.win_tps;
-    const int tact = min(min(ntile, t0 + d.win_tps), (slice_nact + T - 1) / T);
-    const float2* __restrict__ x = reinterpret_cast<const float2*>(d.V) + (size_t)j * d.vs;
-    float* __restrict__ y = reinterpret_cast<float*>(reinterpret_cast<float2*>(d.V) + (size_t)(j + 1) * d.vs);
-    const float* __restrict__ diag = reinterpret_cast<const float*>(d.diag);
-    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)d.win_rec, (short)0, 0x7fffffff, 0x00020000);
-    constexpr uint32_t kRec = 18 * 4;                        // bytes per batch record
-    const uint32_t ldw = 4u * (uint32_t)(lane < 16 ? lane : 16 + (lane & 1));
-    const uint32_t lane_b = 4u * (uint32_t)lane;             // this lane's real column in a ring row
-    const float gc = (float)d.g;
-    // x rows at or beyond the slice's n_active are zero in every slot
-    auto xrow4 = [&](int row, int c) {
-        return row < slice_nact ? *reinterpret_cast<const float4*>(x + (size_t)row * Bp + b0 + 2 * c)
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-    };
-    double acc0 = 0.0, acc1 = 0.0;
-    if (t0 < tact) {
-        // the first tile's whole window [lo', hi'], four loads in flight per thread
-        {
-            const int f0 = d.win_lo[2 * t0], cnt = (d.win_lo[2 * t0 + 1] - f0 + 1) * C4;
-            for (int q0 = tid; q0 < cnt; q0 += 4 * nthr) {
-                float4 v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = q0 + u * nthr;
-                    v[u] = q < cnt ? xrow4(f0 + q / C4, q % C4) : make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int q = q0 + u * nthr;
-                    if (q < cnt) *reinterpret_cast<float4*>(ring + ((f0 + q / C4) % RING) * RW + 4 * (q % C4)) = v[u];
-                }
-            }
-        }
-        __syncthreads();
-        for (int t = t0; t < tact; ++t) {
-            // the next tile's new rows (hi'_t, hi'_{t+1}] are loaded now and
-            // stored after this tile's products; a jump of the window beyond
-            // kPre float4 per thread (a wall boundary) loads the rest then
-            const int n0 = d.win_lo[2 * t + 1] + 1;
-            const int ncnt = t + 1 < tact ? (d.win_lo[2 * (t + 1) + 1] - n0 + 1) * C4 : 0;
-            float4 pre[kPre];
-#pragma unroll
-            for (int u = 0; u < kPre; ++u) {
-                const int q = tid + u * nthr;
-                pre[u] = q < ncnt ? xrow4(n0 + q / C4, q % C4) : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            const int rb1 = min((t + 1) * (T >> 2), nblk);
-            for (int rb = t * (T >> 2) + wave; rb < rb1; rb += nw) {
-                const int s0 = uniform(d.win_gptr[rb]), s1 = uniform(d.win_gptr[rb + 1]);
-                mf4 c = {0.f, 0.f, 0.f, 0.f};
-                if (s1 > s0) {
-                    // records kDepth batches ahead; the LDS reads of batch u + 1
-                    // before the products of batch u
-                    const int bb = s0 >> 2, nb = (s1 - s0) >> 2;
-                    uint32_t rec[kDepth];
-#pragma unroll
-                    for (int u = 0; u < kDepth; ++u) {
-                        rec[u] = __builtin_amdgcn_raw_buffer_load_b32(rr, (uint32_t)(bb + min(u, nb - 1)) * kRec + ldw, 0, 0);
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                    float bv[2][4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        bv[0][k] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(ring) +
-                                                                   ((uint32_t)roll_slot(rec[0], k) << 8) + lane_b);
-                    for (int u = 0; u < nb; u += kDepth) {
-#pragma unroll
-                        for (int w = 0; w < kDepth; ++w) {
-                            const int cur = u + w;
-                            if (cur < nb) {
-                                if (cur + 1 < nb) {
-#pragma unroll
-                                    for (int k = 0; k < 4; ++k)
-                                        bv[(w + 1) & 1][k] = *reinterpret_cast<const float*>(
-                                            reinterpret_cast<const char*>(ring) +
-                                            ((uint32_t)roll_slot(rec[(w + 1) % kDepth], k) << 8) + lane_b);
-                                }
-                                const float a = __uint_as_float(rec[w]);
-                                c = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bv[w & 1][0], c, 4, 0, 0);
-                                c = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bv[w & 1][1], c, 4, 1, 0);
-                                c = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bv[w & 1][2], c, 4, 2, 0);
-                                c = __builtin_amdgcn_mfma_f32_4x4x1f32(a, bv[w & 1][3], c, 4, 3, 0);
-                            }
-                            rec[w] = __builtin_amdgcn_raw_buffer_load_b32(
-                                rr, (uint32_t)(bb + min(cur + kDepth, nb - 1)) * kRec + ldw, 0, 0);
-                            __builtin_amdgcn_sched_barrier(0);
-                        }
-                    }
-                }
-                // rows 4 rb .. 4 rb + 3 of this lane's real column
-                const int b = b0 + (lane >> 1);
-                const int nact_b = d.nact[b];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int row = 4 * rb + i;
-                    const bool valid = row < d.n;
-                    float xv = 0.f, wv = 0.f;
-                    if (valid) {
-                        xv = ring[(row % RING) * RW + lane];
-                        const float dv = diag[(size_t)row * Bp + b];
-                        wv = __builtin_fmaf(gc, c[i], dv * xv);
-                        if (row >= nact_b) wv = 0.f;
-                        if (!last) y[((size_t)row * Bp + b0) * 2 + lane] = wv;
-                    }
-                    // the partner lane holds the other part of the same trajectory
-                    const float xp = __shfl_xor(xv, 1, 64), wp = __shfl_xor(wv, 1, 64);
-                    if (valid && (lane & 1) == 0) {
-                        const double xr = (double)xv, xm = (double)xp, dr = (double)wv, dm = (double)wp;
-                        if (!last) {
-                            acc0 = __builtin_fma(xm, dm, __builtin_fma(xr, dr, acc0));
-                            acc1 = __builtin_fma(-xm, dr, __builtin_fma(xr, dm, acc1));
-                        } else {
-                            acc0 = __builtin_fma(dm, dm, __builtin_fma(dr, dr, acc0));
-                        }
-                    }
-                }
-            }
-            __syncthreads();                                 // every wave is done with tile t
-#pragma unroll
-            for (int u = 0; u < kPre; ++u) {
-                const int q = tid + u * nthr;
-                if (q < ncnt) *reinterpret_cast<float4*>(ring + ((n0 + q / C4) % RING) * RW + 4 * (q % C4)) = pre[u];
-            }
-            for (int q = tid + kPre * nthr; q < ncnt; q += nthr)
-                *reinterpret_cast<float4*>(ring + ((n0 + q / C4) % RING) * RW + 4 * (q % C4)) = xrow4(n0 + q / C4, q % C4);
-            __syncthreads();
-        }
-    }
-    // per trajectory over the waves, in order (the ring is free now)
-    __syncthreads();
-    double* red = reinterpret_cast<double*>(ring);
-    red[(2 * wave) * 64 + lane] = acc0;
-    red[(2 * wave + 1) * 64 + lane] = acc1;
-    __syncthreads();
-    if (wave == 0 && (lane & 1) == 0) {
-        const int t = lane >> 1;
-        for (int q = 0; q < 2; ++q) {
-            if (q == 1 && last) break;
-            double v = 0.0;
-            for (int w = 0; w < nw; ++w) v += red[(2 * w + q) * 64 + lane];
-            d.part_s[((size_t)q * d.ntile_p + strip) * Bp + b0 + t] = v;
-        }
-    }
-}
-
-// LDS bytes a launch needs: the largest window (or ring), and the partials of 16 waves
+// LDS bytes a launch needs: the largest window, and the partials of 16 waves
 size_t spmm_win_lds_bytes(int ts, int max_rows)
 {
     const size_t w = (size_t)max_rows * 2 * ts * sizeof(float);
@@ -394,29 +199,20 @@ __global__ __launch_bounds__(1024) void k_spmm_mfma_win(Dev d, int j, int last)
 hipError_t launch_spmm_win(const Dev& d, int j, int last, hipStream_t s)
 {
     const int ts = d.win_ts;
+    const dim3 grid(wpad8(d.ntile_p * (d.sn / ts))), block(64 * d.win_w);
     const size_t lds = (size_t)d.win_lds;
-    const dim3 block(64 * d.win_w);
-    if (d.rb == 16) {
-        // 16-row blocks on 16 x 16 x 4 MFMAs, one tile window per workgroup
-        const dim3 grid(wpad8(d.ntile_p * (d.sn / ts)));
-        if (ts == 32) hipLaunchKernelGGL((k_spmm_mfma_win<4>), grid, block, lds, s, d, j, last);
-        else hipLaunchKernelGGL((k_spmm_mfma_win<2>), grid, block, lds, s, d, j, last);
-    } else {
-        // 4-row blocks, persistent over strips with the LDS ring
-        const dim3 grid(wpad8(d.win_strips * (d.sn / 32)));
-        hipLaunchKernelGGL(k_spmm_roll, grid, block, lds, s, d, j, last);
-    }
+    if (ts == 32) hipLaunchKernelGGL((k_spmm_mfma_win<4>), grid, block, lds, s, d, j, last);
+    else hipLaunchKernelGGL((k_spmm_mfma_win<2>), grid, block, lds, s, d, j, last);
     return hipGetLastError();
 }
 
-// lift the dynamic LDS limit of both instantiations to `bytes` (once per process)
+// lift the dynamic LDS limit of the instantiations to `bytes` (once per process)
 hipError_t spmm_win_set_lds_limit(size_t bytes)
 {
-    const void* fns[3] = {reinterpret_cast<const void*>(&k_spmm_roll),
-                          reinterpret_cast<const void*>(&k_spmm_mfma_win<2>),
+    const void* fns[2] = {reinterpret_cast<const void*>(&k_spmm_mfma_win<2>),
                           reinterpret_cast<const void*>(&k_spmm_mfma_win<4>)};
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
         e = hipFuncSetAttribute(fns[k], hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     return e;
 }

@@ -228,6 +228,18 @@ dq_status dq_propagate_step(dq_system *sys);
  * hipGraph (one graph launch per dt). */
 dq_status dq_run(dq_system *sys, int64_t nsteps, int32_t use_graph);
 dq_status dq_get_step(dq_system *sys, int64_t *step);
+/* Krylov substeps from the Expokit error estimate (padeClass.cu:749-782,
+ * systemClass.cu:2055-2085; the reference only sums the estimate).  With
+ * tol > 0, dq_run advances in chunks of chunk_steps steps; a chunk is kept
+ * when K times the largest clamped estimate of any Krylov cycle of any
+ * trajectory is at most tol (the error allowed per dt), and is otherwise
+ * redone from its start with twice the substeps, K <= k_max.  K starts at
+ * krylov_substeps of the config and never decreases (the noise is
+ * stationary).  tol <= 0 switches the control off and restores the
+ * configured K.  Lanczos propagator only. */
+dq_status dq_system_set_krylov_control(dq_system *sys, double tol, int32_t k_max, int32_t chunk_steps);
+/* current K and the number of chunks redone so far */
+dq_status dq_get_krylov_control(dq_system *sys, int32_t *k_now, int64_t *rejected_chunks);
 
 /* ---- results ---------------------------------------------------------- */
 /* pops[b][k][g] for records k = 0..step/every (*nrec set to that count). */
@@ -324,16 +336,18 @@ typedef struct dq_tuning {
     int32_t concurrent_slices;   /* 1, 2 or 4: a graph-captured step runs that many equal slices
                                     of the trajectory slots on their own streams (each slice a
                                     multiple of 64 slots) */
-    int32_t spmm_window_rows;    /* 0: the row-blocked SpMMs above.  T > 0 (a multiple of 8):
+    int32_t spmm_window_rows;    /* 0: the SpMMs above.  T > 0 (a multiple of 16, <= 4096):
                                     the LDS-window SpMM (mixed precision only): a workgroup
                                     stages the x rows a tile of T rows references in LDS and
-                                    multiplies 4-row blocks on the matrix cores; needs
-                                    spmm_rows_per_wave = 4, spmm_phases = 1, spmm_pipeline = 0,
+                                    multiplies 16-row blocks on the matrix cores, bit for bit
+                                    the result of spmm_rows_per_wave = 16; needs
+                                    spmm_rows_per_wave = 16, spmm_phases = 1, spmm_pipeline = 0,
                                     spmm_traj_per_lane 1 or 2 (16 or 32 trajectories per wave)
                                     and every tile's window (8 * 16 * spmm_traj_per_lane bytes
                                     per referenced row, a banded matrix keeps it short) within
-                                    160 KiB, else DQ_ERR_INVALID_ARG */
-    int32_t spmm_window_waves;   /* 4, 8 or 16 waves per workgroup of the LDS-window SpMM */
+                                    the device's LDS per workgroup (160 KiB on gfx950), else
+                                    DQ_ERR_INVALID_ARG */
+    int32_t spmm_window_waves;   /* 0 (8), 4, 8 or 16 waves per workgroup of the LDS-window SpMM */
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

@@ -169,6 +169,12 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
                   const int32_t *n_active, int64_t nsteps, int obs_every,
                   const int32_t *group, int ngroups, const double *bra,
                   double *pops, double *corr, double *partic, double *err_sum, int nthreads);
+/* substep control from the Expokit estimate, a batch advanced chunk by chunk */
+int orc_run_controlled(const orc_params *p, const orc_site_params *sp,
+                       const int32_t *rowptr, const int32_t *col, const double *val,
+                       const double *psi0, const uint64_t *traj, int nb, int64_t nsteps,
+                       double tol, int kmax, int chunk, int obs_every, const int32_t *group, int ngroups,
+                       double *pops, double *psi_out, double *err_sum, int32_t *k_of_step, int64_t *rejects);
 
 /* One Taylor-Krylov cycle on psi (taylorKrylovClass.cu:217-434), A = the
  * Hamiltonian (val, g, diag), the auxiliary matrix from p.  Returns the

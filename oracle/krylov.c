@@ -701,3 +701,86 @@ int orc_run_batch(const orc_params *p, const orc_site_params *sp,
     }
     return fail ? -1 : 0;
 }
+
+/* Substep control restated (the engine's dq_system_set_krylov_control): the
+ * batch advances in chunks of `chunk` steps with K Krylov cycles per dt; a
+ * chunk is kept when K times the largest clamped Expokit estimate of any
+ * cycle of any trajectory is at most tol, and is otherwise redone from its
+ * start with 2K cycles (K <= kmax).  K starts at p->krylov_substeps.
+ * Populations every obs_every steps (pops[b][rec][g]), final states
+ * psi_out[b][2n], error sums err_sum[b], the K of every step k_of_step[step]
+ * and the number of redone chunks. */
+int orc_run_controlled(const orc_params *p, const orc_site_params *sp,
+                       const int32_t *rowptr, const int32_t *col, const double *val,
+                       const double *psi0, const uint64_t *traj, int nb, int64_t nsteps,
+                       double tol, int kmax, int chunk, int obs_every, const int32_t *group, int ngroups,
+                       double *pops, double *psi_out, double *err_sum, int32_t *k_of_step, int64_t *rejects)
+{
+    const int n = p->n, m = p->m, nt = m + 2;
+    const size_t cs = 2 * (size_t)n;
+    const size_t nobs = obs_every > 0 ? (size_t)(nsteps / obs_every + 1) : 0;
+    double *psi = (double *)malloc(sizeof(double) * cs * nb * 2);
+    double *psi_ck = psi + cs * nb;
+    double *e0 = (double *)malloc(sizeof(double) * n * nb * 3);
+    double *de = e0 + (size_t)n * nb, *de_ck = de + (size_t)n * nb;
+    double *esum = (double *)calloc(2 * (size_t)nb, sizeof(double));
+    double *esum_ck = esum + nb;
+    double *diag = (double *)malloc(sizeof(double) * n);
+    double *Q = (double *)malloc(sizeof(double) * cs * (m + 2));
+    double *work = (double *)malloc(sizeof(double) * (2 * (m + 2) + nt * nt * 5));
+    orc_lanczos_opts lo = {m, p->reorth, p->breakdown_tol};
+    int rc = 0, K = p->krylov_substeps;
+    *rejects = 0;
+    for (int b = 0; b < nb; ++b) {
+        double *pb = psi + cs * b;
+        memcpy(pb, psi0 + cs * b, cs * sizeof(double));
+        double s2 = 0.0;
+        for (int i = 0; i < n; ++i) s2 += pb[2 * i] * pb[2 * i] + pb[2 * i + 1] * pb[2 * i + 1];
+        double inv = 1.0 / sqrt(s2);
+        for (size_t i = 0; i < cs; ++i) pb[i] *= inv;
+        orc_init_disorder(p, sp, traj[b], e0 + (size_t)n * b, de + (size_t)n * b);
+        if (pops && obs_every > 0) record(n, pb, group, ngroups, pops + (size_t)b * nobs * ngroups);
+    }
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int64_t c = nsteps - done < chunk ? nsteps - done : chunk;
+        memcpy(psi_ck, psi, sizeof(double) * cs * nb);
+        memcpy(de_ck, de, sizeof(double) * n * nb);
+        memcpy(esum_ck, esum, sizeof(double) * nb);
+        const double tau = p->dt_fs / (double)K;
+        double emax = 0.0;
+        for (int b = 0; b < nb; ++b) {
+            double *pb = psi + cs * b, *db = de + (size_t)n * b, *eb = e0 + (size_t)n * b;
+            for (int64_t step = done; step < done + c; ++step) {
+                orc_noise_step(p, sp, traj[b], step, db);
+                for (int i = 0; i < n; ++i) diag[i] = db[i] + eb[i];
+                for (int k = 0; k < K; ++k) {
+                    double err = krylov_cycle(n, rowptr, col, val, p->g, diag, &lo, tau, p->hbar, pb, Q, work);
+                    if (err < 0.0) { rc = -1; goto done_label; }
+                    if (err > emax) emax = err;
+                    esum[b] += err;
+                    double inv = 1.0 / cnorm(n, pb);
+                    for (size_t i = 0; i < cs; ++i) pb[i] *= inv;
+                }
+                if (pops && obs_every > 0 && (step + 1) % obs_every == 0)
+                    record(n, pb, group, ngroups, pops + ((size_t)b * nobs + (step + 1) / obs_every) * ngroups);
+            }
+        }
+        if (!(emax * K <= tol) && 2 * K <= kmax) {
+            memcpy(psi, psi_ck, sizeof(double) * cs * nb);
+            memcpy(de, de_ck, sizeof(double) * n * nb);
+            memcpy(esum, esum_ck, sizeof(double) * nb);
+            K *= 2;
+            ++*rejects;
+            continue;
+        }
+        for (int64_t step = done; step < done + c; ++step)
+            if (k_of_step) k_of_step[step] = K;
+        done += c;
+    }
+done_label:
+    if (psi_out) memcpy(psi_out, psi, sizeof(double) * cs * nb);
+    if (err_sum) memcpy(err_sum, esum, sizeof(double) * nb);
+    free(psi); free(e0); free(esum); free(diag); free(Q); free(work);
+    return rc;
+}

@@ -104,6 +104,11 @@ def lib():
                                     _up, C.c_int, C.c_void_p, C.c_int64, C.c_int, C.c_void_p,
                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_int]
+        L.orc_run_controlled.restype = C.c_int
+        L.orc_run_controlled.argtypes = [C.POINTER(Params), C.POINTER(SiteParams), _ip, _ip, _dp, _dp,
+                                         _up, C.c_int, C.c_int64, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_void_p, C.c_int, C.c_void_p, _dp, _dp, C.c_void_p,
+                                         C.POINTER(C.c_int64)]
         L.orc_fft.restype = C.c_int
         L.orc_fft.argtypes = [C.c_int, _dp, C.c_int]
         L.orc_dft.argtypes = [C.c_int, _dp, _dp, C.c_int]
@@ -396,6 +401,33 @@ def run_batch(p, sites, rowptr, col, val, psi0, traj_ids, nsteps, obs_every, gro
     return dict(pops=pops.reshape(nb, nobs, ngroups),
                 corr=None if corr is None else i2c(corr).reshape(nb, nsteps + 1), err=err,
                 partic=None if pr is None else pr.reshape(nb, nsteps + 1))
+
+
+def run_controlled(p, sites, rowptr, col, val, psi0, traj_ids, nsteps, tol, kmax, chunk, obs_every=0,
+                   group=None, ngroups=0):
+    """substep control restated: the batch advanced chunk by chunk, K doubled
+    while K x the largest cycle estimate exceeds tol"""
+    n = p.n
+    traj_ids = np.asarray(traj_ids, np.uint64)
+    nb = traj_ids.size
+    psi0 = np.asarray(psi0, np.complex128)
+    if psi0.ndim == 1:
+        psi0 = np.broadcast_to(psi0, (nb, n))
+    nobs = nsteps // obs_every + 1 if obs_every > 0 else 0
+    pops = np.zeros(max(nb * nobs * ngroups, 1))
+    psi = np.zeros(2 * n * nb)
+    err = np.zeros(nb)
+    ks = np.zeros(nsteps, np.int32)
+    rej = C.c_int64()
+    grp = None if group is None else _c(np.asarray(group, np.int32))
+    rc = lib().orc_run_controlled(C.byref(p), C.byref(sites.sp), _c(rowptr), _c(col), _c(val),
+                                  c2i(psi0.ravel()), traj_ids, nb, nsteps, tol, kmax, chunk, obs_every,
+                                  _ptr(grp), ngroups, pops.ctypes.data if nobs else None, psi, err,
+                                  ks.ctypes.data, C.byref(rej))
+    if rc != 0:
+        raise RuntimeError("orc_run_controlled failed")
+    return dict(pops=pops[:nb * nobs * ngroups].reshape(nb, nobs, ngroups) if nobs else None,
+                psi=i2c(psi).reshape(nb, n), err=err, k_of_step=ks, rejects=int(rej.value))
 
 
 def participation(psi):
