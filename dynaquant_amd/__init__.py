@@ -34,6 +34,7 @@ EXPORTS = [
     "dq_read_position_file", "dq_cd_vectors", "dq_system_set_participation", "dq_get_participation",
     "dq_system_set_taylor_krylov", "dq_system_snapshot", "dq_system_restore", "dq_system_set_bras",
     "dq_spectrum_weighted", "dq_system_set_krylov_control", "dq_get_krylov_control",
+    "dq_spectrum_noalloc",
 ]
 
 KERNEL_CLASSES = ["prologue", "spmm", "scalars", "update", "reduce", "spmm_last", "pade",
@@ -399,6 +400,14 @@ class System:
         om = np.zeros(nfft)
         sp = np.zeros(2 * nfft)
         _check(lib().dq_spectrum(self._h, C.c_double(sigma_fs), C.c_double(mu2), nfft, _p(om), _p(sp)))
+        return om, sp.view(np.complex128)
+
+    def spectrum_noalloc(self, sigma_fs, mu2, nfft):
+        """spectrum() in the library's fixed device workspace (nfft <= 131072)"""
+        om = np.zeros(nfft)
+        sp = np.zeros(2 * nfft)
+        _check(lib().dq_spectrum_noalloc(self._h, C.c_double(sigma_fs), C.c_double(mu2), nfft, _p(om),
+                                         _p(sp)))
         return om, sp.view(np.complex128)
 
 

@@ -1712,8 +1712,11 @@ dq_status dq_fft(int32_t n, int32_t batch, double* data, int32_t sign)
 // Spectrum of C(n) = (wts ? sum_b wts[b] C_b(n) : batch mean) times scale / T:
 // window, zero padding to nfft, inverse-direction FFT on the GPU
 // (spectraClass.cu:575-750).  wts_host: per caller trajectory, or null.
+// work: null (the buffers are allocated here), or 2 * nfft complex doubles of
+// device memory for the windowed C(n) and the FFT's second buffer (no
+// weights then): every element is written before it is read.
 static dq_status spectrum_impl(dq_system* sys, double sigma_fs, double scale_mu2, const double* wts_host,
-                               int32_t nfft, double* omega, double* spec)
+                               int32_t nfft, double* omega, double* spec, double2* work = nullptr)
 {
     if (!sys || !omega || !spec) return fail(DQ_ERR_INVALID_ARG, "null argument");
     if (!sys->have_bra) return fail(DQ_ERR_STATE, "no correlation function (bra not set)");
@@ -1731,9 +1734,14 @@ static dq_status spectrum_impl(dq_system* sys, double sigma_fs, double scale_mu2
     double2 *a = nullptr, *b = nullptr;
     double* w = nullptr;
     const size_t bytes = sizeof(double2) * (size_t)nfft;
-    hipError_t e;
-    if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess ||
-        (wts_host && (e = hipMalloc(&w, sizeof(double) * wslot.size())) != hipSuccess)) {
+    hipError_t e = hipSuccess;
+    const bool own = work == nullptr;
+    if (!own) {
+        if (wts_host) return fail(DQ_ERR_INVALID_ARG, "weights need allocated buffers");
+        a = work;
+        b = work + nfft;
+    } else if ((e = hipMalloc(&a, bytes)) != hipSuccess || (e = hipMalloc(&b, bytes)) != hipSuccess ||
+               (wts_host && (e = hipMalloc(&w, sizeof(double) * wslot.size())) != hipSuccess)) {
         hipFree(a); hipFree(b); hipFree(w);
         return fail(DQ_ERR_OOM, hipGetErrorString(e));
     }
@@ -1742,7 +1750,7 @@ static dq_status spectrum_impl(dq_system* sys, double sigma_fs, double scale_mu2
     if (e == hipSuccess) e = dq::launch_fft(nfft, 1, a, b, +1, sys->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
     if (e == hipSuccess) e = hipMemcpy(spec, a, bytes, hipMemcpyDeviceToHost);
-    hipFree(a); hipFree(b); hipFree(w);
+    if (own) { hipFree(a); hipFree(b); hipFree(w); }
     if (e != hipSuccess) return fail(DQ_ERR_HIP, hipGetErrorString(e));
     // spectraClass.cu:147-151 (true lightspeed on the frequency axis) and
     // the |mu|^2 / T scaling of spectraClass.cu:641-686
@@ -1762,6 +1770,19 @@ extern "C" {
 dq_status dq_spectrum(dq_system* sys, double sigma_fs, double mu2, int32_t nfft, double* omega, double* spec)
 {
     return spectrum_impl(sys, sigma_fs, mu2, nullptr, nfft, omega, spec);
+}
+
+// dq_spectrum in the library's fixed device workspace (dq_noalloc.hip): no
+// device memory is allocated or freed.  Calls that overlap on one GPU share
+// that workspace and race.
+dq_status dq_spectrum_noalloc(dq_system* sys, double sigma_fs, double mu2, int32_t nfft, double* omega, double* spec)
+{
+    if (!sys || !omega || !spec) return fail(DQ_ERR_INVALID_ARG, "null argument");
+    if (nfft > dq::kNoAllocMaxNfft)
+        return fail(DQ_ERR_INVALID_ARG, "nfft exceeds the fixed workspace of dq_spectrum_noalloc (131072)");
+    double2* work = nullptr;
+    HIP_TRY(dq::noalloc_scratch(&work));
+    return spectrum_impl(sys, sigma_fs, mu2, nullptr, nfft, omega, spec, work);
 }
 
 dq_status dq_spectrum_weighted(dq_system* sys, double sigma_fs, const double* weight, int32_t nfft, double* omega,

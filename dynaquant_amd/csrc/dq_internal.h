@@ -158,5 +158,9 @@ hipError_t launch_tk_start(const Dev& d, hipStream_t s);
 hipError_t launch_tk_combine(const Dev& d, hipStream_t s);
 hipError_t launch_spectrum_prep(const Dev& d, int64_t T, double dt, double sigma, int nfft,
                                 double2* buf, const double* wts, hipStream_t s);
+// dq_noalloc.hip: the current device's fixed workspace of 2 * kNoAllocMaxNfft
+// complex doubles (its address, looked up once per device)
+constexpr int kNoAllocMaxNfft = 1 << 17;
+hipError_t noalloc_scratch(double2** out);
 
 }  // namespace dq

@@ -279,6 +279,15 @@ dq_status dq_spectrum(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
  * dq_spectrum is the case weight[b] = mu2 / B). */
 dq_status dq_spectrum_weighted(dq_system *sys, double sigma_fs, const double *weight, int32_t nfft,
                                double *omega, double *spec);
+/* No counterpart in the reference: dq_spectrum for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The windowed C(n) and the FFT's second
+ * buffer live in a fixed per-GPU workspace of 2 * 131072 complex doubles
+ * (4 MiB, held until the process ends): nfft <= 131072, else
+ * DQ_ERR_INVALID_ARG.  Same result as dq_spectrum, bit for bit.  Calls that
+ * overlap on one GPU share the workspace and race. */
+dq_status dq_spectrum_noalloc(dq_system *sys, double sigma_fs, double mu2, int32_t nfft,
+                              double *omega, double *spec);
 /* In-place FFTs of `batch` interleaved complex rows of length n (power of
  * two, <= 65536) on the GPU (replaces cufftExecZ2Z, spectraClass.cu:617). */
 dq_status dq_fft(int32_t n, int32_t batch, double *data, int32_t sign);
