@@ -62,12 +62,12 @@ class Tuning(C.Structure):
         ("spmm_rows_per_wave", C.c_int32), ("spmm_traj_per_lane", C.c_int32),
         ("spmm_phases", C.c_int32), ("tile_rows", C.c_int32), ("spmm_pipeline", C.c_int32),
         ("concurrent_slices", C.c_int32), ("spmm_window_rows", C.c_int32),
-        ("spmm_window_waves", C.c_int32),
+        ("spmm_window_waves", C.c_int32), ("spmm_exact_sums", C.c_int32),
     ]
 
 
 TUNING_KEYS = ["spmm_rows_per_wave", "spmm_traj_per_lane", "spmm_phases", "tile_rows", "spmm_pipeline",
-               "concurrent_slices", "spmm_window_rows", "spmm_window_waves"]
+               "concurrent_slices", "spmm_window_rows", "spmm_window_waves", "spmm_exact_sums"]
 
 
 class TubeSpec(C.Structure):

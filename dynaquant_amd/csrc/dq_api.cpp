@@ -68,6 +68,7 @@ struct RowBlocks {
     int32_t* col = nullptr;
     float* val32 = nullptr;
     double* val64 = nullptr;
+    uint16_t* lo16 = nullptr;   // R = 16: bfloat16 low parts val - (double)val32, MFMA operand order
 };
 
 // LDS one workgroup of the window SpMM may use: the device's opt-in limit
@@ -241,6 +242,16 @@ static void release_row_blocks(dq_system* sys, RowBlocks* b)
     sys->release(b->col);
     sys->release(b->val32);
     sys->release(b->val64);
+    sys->release(b->lo16);
+}
+
+// bfloat16 bits of f, rounded to nearest even (f finite)
+static uint16_t bf16_bits(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
 }
 
 // the matrix-core SpMM reads x rows and dense block values through buffer
@@ -291,6 +302,23 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
                                         "64 * union slots must stay below 2^31 bytes");
     std::vector<float> rbv32(rbv64.size());
     for (size_t q = 0; q < rbv64.size(); ++q) rbv32[q] = (float)rbv64[q];
+    // R = 16: the FP32 values are a fixed perturbation of the couplings (up to
+    // half an FP32 ulp each), which the matrix-core SpMM takes back with one
+    // v_mfma_f32_16x16x16bf16 per 16 union slots on the low parts
+    // val - (double)val32, stored in that instruction's operand order: group G
+    // of slots 16 G .. 16 G + 15, lane i + 16 kq, element q = row i, slot
+    // 16 G + 4 q + kq (the slot lane group kq multiplies in K-step q)
+    std::vector<uint16_t> lo16;
+    if (R == 16) {
+        const size_t ng = rbc.size() / 16;
+        lo16.assign(std::max<size_t>(ng, 1) * 256, 0);
+        for (size_t G = 0; G < ng; ++G)
+            for (int l = 0; l < 64; ++l)
+                for (int q = 0; q < 4; ++q) {
+                    const size_t e = (16 * G + 4 * q + (l >> 4)) * 16 + (l & 15);
+                    lo16[G * 256 + 4 * l + q] = bf16_bits((float)(rbv64[e] - (double)rbv32[e]));
+                }
+    }
     // the new blocks are complete before the old ones are released
     Pending pend(sys);
     RowBlocks nb;
@@ -298,11 +326,15 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
     DQ_TRY(pend.alloc(&nb.col, std::max<size_t>(rbc.size(), 1)));
     DQ_TRY(pend.alloc(&nb.val32, std::max<size_t>(rbv32.size(), 1)));
     DQ_TRY(pend.alloc(&nb.val64, std::max<size_t>(rbv64.size(), 1)));
+    if (R == 16) DQ_TRY(pend.alloc(&nb.lo16, lo16.size()));
     HIP_TRY(hipMemcpyAsync(nb.ptr, rbp.data(), sizeof(int32_t) * rbp.size(), hipMemcpyHostToDevice, sys->stream));
     if (!rbc.empty()) {
         HIP_TRY(hipMemcpyAsync(nb.col, rbc.data(), sizeof(int32_t) * rbc.size(), hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(hipMemcpyAsync(nb.val32, rbv32.data(), sizeof(float) * rbv32.size(), hipMemcpyHostToDevice, sys->stream));
         HIP_TRY(hipMemcpyAsync(nb.val64, rbv64.data(), sizeof(double) * rbv64.size(), hipMemcpyHostToDevice, sys->stream));
+        if (R == 16)
+            HIP_TRY(hipMemcpyAsync(nb.lo16, lo16.data(), sizeof(uint16_t) * lo16.size(), hipMemcpyHostToDevice,
+                                   sys->stream));
     }
     HIP_TRY(hipStreamSynchronize(sys->stream));
     pend.keep();
@@ -345,8 +377,9 @@ static dq_status build_row_blocks(dq_system* sys, int R)
     old.col = const_cast<int32_t*>(d.rb_col);
     old.val32 = const_cast<float*>(d.rb_val32);
     old.val64 = const_cast<double*>(d.rb_val64);
+    old.lo16 = const_cast<uint16_t*>(d.rb_lo16);
     release_row_blocks(sys, &old);
-    d.rb_ptr = a.ptr; d.rb_col = a.col; d.rb_val32 = a.val32; d.rb_val64 = a.val64;
+    d.rb_ptr = a.ptr; d.rb_col = a.col; d.rb_val32 = a.val32; d.rb_val64 = a.val64; d.rb_lo16 = a.lo16;
     d.rb = R;
     refresh_spmm_tiles(sys);
     return DQ_OK;
@@ -660,10 +693,7 @@ static void release_tk(dq_system* sys)
     sys->release(sys->tk.col);
     sys->release(sys->tk.val64);
     sys->release(sys->tk.val32);
-    sys->release(sys->tk.rb.ptr);
-    sys->release(sys->tk.rb.col);
-    sys->release(sys->tk.rb.val32);
-    sys->release(sys->tk.rb.val64);
+    release_row_blocks(sys, &sys->tk.rb);
     sys->release(sys->tk.diag);
     sys->release(d.tk_t);
     sys->release(d.tk_u);
@@ -1215,6 +1245,7 @@ static Dev tk_dev(const dq_system* sys, const Dev& d)
     a.rb_col = sys->tk.rb.col;
     a.rb_val32 = sys->tk.rb.val32;
     a.rb_val64 = sys->tk.rb.val64;
+    a.rb_lo16 = sys->tk.rb.lo16;
     a.diag = sys->tk.diag;
     a.g = 1.0;
     // the auxiliary matrix goes through the row-blocked SpMM
@@ -2106,6 +2137,10 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "spmm_pipeline must be 0 or 1";
         return false;
     }
+    if (t.spmm_exact_sums != 0 && t.spmm_exact_sums != 1) {
+        *why = "spmm_exact_sums must be 0 or 1";
+        return false;
+    }
     const int ns = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
     if (ns != 1 && ns != 2 && ns != 4) {
         *why = "concurrent_slices must be 1, 2 or 4";
@@ -2151,6 +2186,7 @@ static void get_tuning(const dq_system* sys, dq_tuning* t)
     t->spmm_phases = sys->d.spmm_phases;
     t->tile_rows = sys->d.plain ? 0 : sys->d.rt_e;
     t->spmm_pipeline = sys->d.spmm_pipe;
+    t->spmm_exact_sums = sys->d.spmm_exact;
     t->concurrent_slices = sys->slices;
     t->spmm_window_rows = sys->win.T;
     t->spmm_window_waves = sys->win.T > 0 ? sys->win_waves : 0;
@@ -2192,6 +2228,7 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     d.spmm_tpl = t.spmm_traj_per_lane;
     d.spmm_phases = t.spmm_phases;
     d.spmm_pipe = t.spmm_pipeline;
+    d.spmm_exact = t.spmm_exact_sums;
     sys->slices = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
     d.rt_e = t.tile_rows;
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
@@ -2268,8 +2305,8 @@ static std::string tuning_key(const dq_system* sys)
         for (char& c : arch) if (c == ' ') c = '_';
     }
     char buf[256];
-    std::snprintf(buf, sizeof(buf), "%s cu=%d n=%d nnz=%lld bp=%d m=%d prec=%s", arch.c_str(), cus, sys->n,
-                  (long long)sys->col.size(), sys->Bp, sys->m, sys->d.mixed ? "mixed" : "fp64");
+    std::snprintf(buf, sizeof(buf), "%s cu=%d n=%d nnz=%lld bp=%d m=%d prec=%s exact=%d", arch.c_str(), cus, sys->n,
+                  (long long)sys->col.size(), sys->Bp, sys->m, sys->d.mixed ? "mixed" : "fp64", sys->d.spmm_exact);
     return buf;
 }
 
@@ -2312,9 +2349,10 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 dq_tuning t;
                 std::memset(&t, 0, sizeof(t));
                 std::string why;
-                if (std::sscanf(line + key.size(), "%d %d %d %d %d %d %d %d", &t.spmm_rows_per_wave,
+                if (std::sscanf(line + key.size(), "%d %d %d %d %d %d %d %d %d", &t.spmm_rows_per_wave,
                                 &t.spmm_traj_per_lane, &t.spmm_phases, &t.tile_rows, &t.spmm_pipeline,
-                                &t.concurrent_slices, &t.spmm_window_rows, &t.spmm_window_waves) == 8 &&
+                                &t.concurrent_slices, &t.spmm_window_rows, &t.spmm_window_waves,
+                                &t.spmm_exact_sums) == 9 &&
                     tuning_valid(sys, t, &why)) {
                     best = t;
                     hit = true;
@@ -2347,7 +2385,9 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     if (e == hipSuccess) e = hipMemcpyAsync(saved_de, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(saved_diag, d.diag, diag_bytes, hipMemcpyDeviceToDevice, sys->stream);
     // candidates: the row-blocked SpMMs, then (mixed precision) the
-    // LDS-window SpMM, grouped so that each list is built once
+    // LDS-window SpMM, grouped so that each list is built once; the caller's
+    // spmm_exact_sums is an accuracy choice and is kept
+    const int32_t exact = orig.spmm_exact_sums;
     std::vector<dq_tuning> cands;
     static const int rows[4] = {128, 64, 256, 32};
     for (int R : {4, 2, 8, 16})
@@ -2364,6 +2404,7 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                     t.spmm_phases = ph;
                     t.tile_rows = rows[cand % 4];
                     t.spmm_pipeline = cand / 4;
+                    t.spmm_exact_sums = exact;
                     cands.push_back(t);
                 }
     if (d.mixed)
@@ -2379,6 +2420,7 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                         t.tile_rows = rows[cand];
                         t.spmm_window_rows = T;
                         t.spmm_window_waves = W;
+                        t.spmm_exact_sums = exact;
                         cands.push_back(t);
                     }
     for (const dq_tuning& t : cands) {
@@ -2450,9 +2492,10 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     DQ_TRY(apply_tuning(sys, best));
     if (cache_path) {
         if (FILE* f = std::fopen(cache_path, "a")) {
-            std::fprintf(f, "%s %d %d %d %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave,
+            std::fprintf(f, "%s %d %d %d %d %d %d %d %d %d\n", key.c_str(), best.spmm_rows_per_wave,
                          best.spmm_traj_per_lane, best.spmm_phases, best.tile_rows, best.spmm_pipeline,
-                         best.concurrent_slices, best.spmm_window_rows, best.spmm_window_waves);
+                         best.concurrent_slices, best.spmm_window_rows, best.spmm_window_waves,
+                         best.spmm_exact_sums);
             std::fclose(f);
         }
     }

@@ -62,7 +62,11 @@ __device__ __forceinline__ void bstore_partials(const double (&acc)[NQ], int nq,
 }
 
 // ---------------------------------------------------------------------------
-// SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory).
+// SpMM  w = d .* x + g * J x  (lanczosClass.cu:618-661 per trajectory), with
+// the FP64 couplings and FP64 sums also for FP32 vectors (x widened exactly,
+// w rounded once): FP32 couplings are a fixed perturbation of H, and FP32 sums
+// over a row's ~60 couplings, whose partial sums cancel to a much smaller w,
+// were the largest part of the mixed-precision population error.
 // Row-blocked: a wave owns kRB consecutive rows (2, 4 or 8, chosen by the
 // autotuner) and walks the union of their columns, so each gathered x run (a
 // contiguous 512 B / 1 KB segment of trajectories) feeds kRB rows from
@@ -101,20 +105,21 @@ __device__ __forceinline__ void gatherg(const V2* __restrict__ xs, const int32_t
     }
 }
 
-// rows r of the block += value(p+u, r) * x(p+u), in column order
+// rows r of the block += value(p+u, r) * x(p+u), in column order, in FP64
+// (FP64 values; an FP32 x is widened exactly)
 template <typename VT, int TPL, int kRB, int G>
-__device__ __forceinline__ void fmag(const VT* __restrict__ rv, int p, const VT (&xr)[G][TPL],
-                                     const VT (&xi)[G][TPL], VT (&ar)[TPL][kRB], VT (&ai)[TPL][kRB])
+__device__ __forceinline__ void fmag(const double* __restrict__ rv, int p, const VT (&xr)[G][TPL],
+                                     const VT (&xi)[G][TPL], double (&ar)[TPL][kRB], double (&ai)[TPL][kRB])
 {
 #pragma unroll
     for (int u = 0; u < G; ++u) {
 #pragma unroll
         for (int r = 0; r < kRB; ++r) {
-            const VT v = rv[(size_t)(p + u) * kRB + r];
+            const double v = rv[(size_t)(p + u) * kRB + r];
 #pragma unroll
             for (int t = 0; t < TPL; ++t) {
-                ar[t][r] = fmav(v, xr[u][t], ar[t][r]);
-                ai[t][r] = fmav(v, xi[u][t], ai[t][r]);
+                ar[t][r] = __builtin_fma(v, (double)xr[u][t], ar[t][r]);
+                ai[t][r] = __builtin_fma(v, (double)xi[u][t], ai[t][r]);
             }
         }
     }
@@ -137,9 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
     V2* __restrict__ y = reinterpret_cast<V2*>(d.V) + (size_t)(j + 1) * d.vs;
     const V2* __restrict__ x = reinterpret_cast<const V2*>(d.V) + (size_t)j * d.vs;
     const VT* __restrict__ diag = reinterpret_cast<const VT*>(d.diag);
-    const VT* __restrict__ rv;
-    if constexpr (sizeof(VT) == 4) rv = d.rb_val32;
-    else rv = d.rb_val64;
+    const double* __restrict__ rv = d.rb_val64;
     const int32_t* __restrict__ rc = d.rb_col;
     const int nblk = (d.n + kRB - 1) / kRB;
     const int rb = rtile * 4 + wave;
@@ -151,11 +154,11 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
     const int slice_nact = uniform(d.nact[ttile * TW]);
     if (rb < nblk && rb * kRB < slice_nact) {
         const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
-        VT ar[TPL][kRB], ai[TPL][kRB];
+        double ar[TPL][kRB], ai[TPL][kRB];
 #pragma unroll
         for (int t = 0; t < TPL; ++t)
 #pragma unroll
-            for (int r = 0; r < kRB; ++r) { ar[t][r] = 0; ai[t][r] = 0; }
+            for (int r = 0; r < kRB; ++r) { ar[t][r] = 0.0; ai[t][r] = 0.0; }
         int p = u0;
         if constexpr (PIPE) {
             // two 4-column register buffers (the registers of one 8-column
@@ -220,15 +223,15 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
             }
 #pragma unroll
             for (int r = 0; r < kRB; ++r) {
-                const VT v = rv[(size_t)p * kRB + r];
+                const double v = rv[(size_t)p * kRB + r];
 #pragma unroll
                 for (int t = 0; t < TPL; ++t) {
-                    ar[t][r] = fmav(v, xr[t], ar[t][r]);
-                    ai[t][r] = fmav(v, xi[t], ai[t][r]);
+                    ar[t][r] = __builtin_fma(v, (double)xr[t], ar[t][r]);
+                    ai[t][r] = __builtin_fma(v, (double)xi[t], ai[t][r]);
                 }
             }
         }
-        const VT g = (VT)d.g;
+        const double g = d.g;
 #pragma unroll
         for (int r = 0; r < kRB; ++r) {
             const int i = rb * kRB + r;
@@ -238,9 +241,10 @@ __global__ __launch_bounds__(kThreads) void k_spmm_b(Dev d, int j, int last, int
                     const int b = b0 + t;
                     const size_t e = (size_t)i * Bp + b;
                     const V2 xv = x[e];
-                    const VT di = diag[e];
-                    VT wr = fmav(g, ar[t][r], di * xv.x);
-                    VT wi = fmav(g, ai[t][r], di * xv.y);
+                    const double di = (double)diag[e];
+                    // w = g J x + d x in FP64, rounded once to the vector type
+                    VT wr = (VT)__builtin_fma(g, ar[t][r], di * (double)xv.x);
+                    VT wi = (VT)__builtin_fma(g, ai[t][r], di * (double)xv.y);
                     if (i >= d.nact[b]) { wr = 0; wi = 0; }
                     if (!last) {
                         V2 w;
@@ -295,7 +299,15 @@ static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, i
 // gathered rows of x: for a slice of 8 NG trajectories a (16 x U) x (U x 16 NG)
 // FP32 product, the 16 NG columns being re/im of the trajectories.  One wave
 // per row block and slice; a K-step of 4 union columns is NG
-// v_mfma_f32_16x16x4_f32.  MFMA lanes: A[i][k] from lane i + 16 k, B[k][c]
+// v_mfma_f32_16x16x4_f32.  Every group of 16 union slots (4 K-steps) adds
+// the low parts of its couplings (val - FP32 val, one
+// v_mfma_f32_16x16x16bf16 on bfloat16 operands), and the FP32 sums move into
+// FP64 ones every two groups, or (EXACT, spmm_exact_sums) after every K-step:
+// the FP32 rounding of the couplings (a fixed perturbation of H) does not
+// enter w, nor an FP32 running sum over the whole union (whose partial sums
+// cancel to the much smaller w); w is rounded to FP32 once.  Over 5000 steps
+// of the 12,000-site tube, the FP32 sums of 32 slots leave 1.6e-6 in the
+// populations, those of one K-step 1.7e-7, at about a third more SpMM time.  MFMA lanes: A[i][k] from lane i + 16 k, B[k][c]
 // from lane c + 16 k, D[4 q + r][c] in register r of lane c + 16 q.  Group g
 // holds the real columns NG c + g, so a lane loads NG contiguous floats of one
 // x row per step and ends with rows 4 q .. 4 q + 3 of the trajectories
@@ -303,13 +315,28 @@ static void launch_spmm_rb(const Dev& d, int j, int last, int tpl, int phases, i
 // ahead of the multiplies (column offset, value and x row of a step are
 // refilled as soon as the step is consumed).  Each output element is the same
 // sequence of MFMA accumulations whatever NG, so the slice width does not
-// change a bit of the result.  Against the 4-row VALU kernel: half the
-// gathered bytes per row at 2.7x the multiply-adds, which run on the matrix
-// cores.  On the 12,000-site dipole tube (B = 256) it takes 39 us against
-// 43 us for that kernel (event-timed, whole batch), and inside the captured
-// step it leaves the vector ALU to the other slices' updates.
+// change a bit of the result.  Against the 4-row kernel: half the gathered
+// bytes per row at 2.7x the multiply-adds, which run on the matrix cores.  On
+// the 12,000-site dipole tube (B = 256) it takes 43 us (45 us with the FP64
+// sums per K-step) against 57-87 us for the row-blocked kernels with their
+// FP64 arithmetic (event-timed, whole batch), and inside the captured step it
+// leaves the vector ALU to the other slices' updates.
 // ---------------------------------------------------------------------------
 typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+// two bfloat16 (the high halves of a and b, truncated) in one dword: a low
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b)
+{
+    return __builtin_amdgcn_perm(__float_as_uint(b), __float_as_uint(a), 0x07060302u);
+}
+
+// low-part operand of the 16 union slots from slot p (a multiple of 16): the
+// four bfloat16 of this lane at byte 512 (p / 16) + 8 lane
+__device__ __forceinline__ bf16x4 lo_at(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int p)
+{
+    return __builtin_bit_cast(bf16x4, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, 32 * p, 0));
+}
 
 template <int NG> struct XV;
 template <> struct XV<2> { using T = float2; };
@@ -358,7 +385,7 @@ __device__ __forceinline__ float xv_at(const typename XV<NG>::T& v, int g)
     }
 }
 
-template <int NG>
+template <int NG, bool EXACT>
 __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
 {
     constexpr int TW = 8 * NG;                       // trajectories per wave
@@ -387,6 +414,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
     const __amdgpu_buffer_rsrc_t ar = __builtin_amdgcn_make_buffer_rsrc((void*)d.rb_val32, (short)0, 0x7fffffff,
                                                                         0x00020000);
     const uint32_t alane = 4u * lane;
+    const __amdgpu_buffer_rsrc_t lr = __builtin_amdgcn_make_buffer_rsrc((void*)d.rb_lo16, (short)0, 0x7fffffff,
+                                                                        0x00020000);
+    const uint32_t llane = 8u * lane;
     const int32_t* __restrict__ rc = d.rb_col;
     const uint32_t rowb = 8u * Bp;                   // bytes per row of x
     const int nblk = (d.n + 15) / 16;
@@ -398,9 +428,18 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
     const int slice_nact = uniform(d.nact[ttile * TW]);
     if (rb < nblk && rb * 16 < slice_nact) {
         const int u0 = uniform(d.rb_ptr[rb]), u1 = uniform(d.rb_ptr[rb + 1]);
+        // FP32 sums of kSumGroups groups of 16 union slots, then FP64 sums:
+        // rows 4 kq + r, real column NG nc + g
+        constexpr int kSumGroups = 2;
+        int ng = 0;
         mfma_f4 c[NG];
+        double s64[NG][4];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+        for (int g = 0; g < NG; ++g) {
+            c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) s64[g][r] = 0.0;
+        }
         // column offsets come through the scalar cache, wave-uniform, 4 kDepth
         // (one 64-byte line) per load, issued one loop iteration before they
         // are used: a vector load there would make every gather wait for all
@@ -432,6 +471,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
         if (u1 > u0) {
             float av[kDepth];
             XT xv[kDepth];
+            bf16x4 alo = lo_at(lr, llane, u0);
             {
                 const Cols c0 = cols(u0);
 #pragma unroll
@@ -447,22 +487,57 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
             for (int base = u0; base < u1; base += 4 * kDepth) {
                 const int pn2 = base + 8 * kDepth < u1 ? base + 8 * kDepth : u0;
                 const Cols nxt = cols(pn2);
+                const bf16x4 alo_n = lo_at(lr, llane, pnext);
+                // the group's x values as the bfloat16 operand of the low parts
+                // (lane group kq: the slot it multiplied in step q; truncated,
+                // which costs the low parts, themselves below 2^-24 of a
+                // coupling, 2^-8 of their size)
+                float xq[NG][kDepth];
 #pragma unroll
                 for (int q = 0; q < kDepth; ++q) {
 #pragma unroll
-                    for (int g = 0; g < NG; ++g)
+                    for (int g = 0; g < NG; ++g) {
                         c[g] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q], xv_at<NG>(xv[q], g), c[g], 0, 0, 0);
+                        xq[g][q] = xv_at<NG>(xv[q], g);
+                    }
+                    if constexpr (EXACT) {
+                        // each K-step's FP32 sum of 4 products into the FP64 sums
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) s64[g][r] += (double)c[g][r];
+                            c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+                        }
+                    }
                     av[q] = a_at(ar, alane, pnext + 4 * q);
                     xv[q] = load_xv<NG>(xr, x_off(pick(cur.s[q])));
                     // step order kept: the refill of step q issues before the
                     // multiplies of step q + 1, so kDepth steps stay in flight
                     __builtin_amdgcn_sched_barrier(0);
                 }
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const uint32_t lo2 = bf16_pair(xq[g][0], xq[g][1]), hi2 = bf16_pair(xq[g][2], xq[g][3]);
+                    const bf16x4 xb = __builtin_bit_cast(bf16x4, (uint2){lo2, hi2});
+                    c[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, xb, c[g], 0, 0, 0);
+                }
+                // every kSumGroups groups (and after the last) the FP32 sums
+                // move into the FP64 ones
+                if (EXACT || ++ng == kSumGroups || base + 4 * kDepth >= u1) {
+                    ng = 0;
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) s64[g][r] += (double)c[g][r];
+                        c[g] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+                alo = alo_n;
                 pnext = pn2;
                 cur = nxt;
             }
         }
-        const float g = (float)d.g;
+        const double g = d.g;
         const float* __restrict__ diag = reinterpret_cast<const float*>(d.diag);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -479,8 +554,9 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
                 }
 #pragma unroll
                 for (int t = 0; t < TL; ++t) {
-                    float wr = fmav(g, c[2 * t][i], dv[t] * xs[2 * t]);
-                    float wi = fmav(g, c[2 * t + 1][i], dv[t] * xs[2 * t + 1]);
+                    // w = g J x + d x in FP64, rounded once to FP32
+                    float wr = (float)__builtin_fma(g, s64[2 * t][i], (double)dv[t] * (double)xs[2 * t]);
+                    float wi = (float)__builtin_fma(g, s64[2 * t + 1][i], (double)dv[t] * (double)xs[2 * t + 1]);
                     if (r >= d.nact[b0 + t]) { wr = 0.f; wi = 0.f; }
                     w[2 * t] = wr;
                     w[2 * t + 1] = wi;
@@ -535,9 +611,15 @@ hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
         const int tl = d.spmm_tpl;
         const int tw = 16 * tl;
         const dim3 grid(bpad8(d.ntile_s * (d.sn / tw))), block(kThreads);
-        if (tl == 4) hipLaunchKernelGGL((k_spmm_mfma<8>), grid, block, 0, s, d, j, last);
-        else if (tl == 2) hipLaunchKernelGGL((k_spmm_mfma<4>), grid, block, 0, s, d, j, last);
-        else hipLaunchKernelGGL((k_spmm_mfma<2>), grid, block, 0, s, d, j, last);
+        if (d.spmm_exact) {
+            if (tl == 4) hipLaunchKernelGGL((k_spmm_mfma<8, true>), grid, block, 0, s, d, j, last);
+            else if (tl == 2) hipLaunchKernelGGL((k_spmm_mfma<4, true>), grid, block, 0, s, d, j, last);
+            else hipLaunchKernelGGL((k_spmm_mfma<2, true>), grid, block, 0, s, d, j, last);
+        } else {
+            if (tl == 4) hipLaunchKernelGGL((k_spmm_mfma<8, false>), grid, block, 0, s, d, j, last);
+            else if (tl == 2) hipLaunchKernelGGL((k_spmm_mfma<4, false>), grid, block, 0, s, d, j, last);
+            else hipLaunchKernelGGL((k_spmm_mfma<2, false>), grid, block, 0, s, d, j, last);
+        }
         return hipGetLastError();
     }
     // trajectory slices of 64 * TPL; the slices run as `phases` launches of

@@ -29,6 +29,7 @@ struct Dev {
     int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
     int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
     int spmm_pipe;                  // batched SpMM: 1 = two gather buffers in flight
+    int spmm_exact;                 // matrix-core SpMM: 1 = FP64 sums after every K-step
     int s0, sn;                     // batched kernels work on trajectory slots [s0, s0 + sn)
     int S;                          // OU substeps
     int noise_model;                // DQ_NOISE_*: 0 OU, 1 HSR, 2 ZZ real, 3 none
@@ -53,6 +54,7 @@ struct Dev {
     const int32_t* rb_col;          // [total union columns]
     const float* rb_val32;          // [total][rb]
     const double* rb_val64;         // [total][rb]
+    const uint16_t* rb_lo16;        // rb = 16: bfloat16 low parts of rb_val32 (16-slot MFMA operands)
     // LDS-window SpMM (dq_window.hip), mixed precision: tiles of win_T rows,
     // slices of win_ts trajectories, win_w waves per workgroup
     int win_T, win_ts, win_w, win_lds;

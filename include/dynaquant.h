@@ -357,6 +357,12 @@ typedef struct dq_tuning {
                                     the device's LDS per workgroup (160 KiB on gfx950), else
                                     DQ_ERR_INVALID_ARG */
     int32_t spmm_window_waves;   /* 0 (8), 4, 8 or 16 waves per workgroup of the LDS-window SpMM */
+    int32_t spmm_exact_sums;     /* matrix-core SpMMs (mixed precision; the row-blocked ones
+                                    always sum in FP64): 0 = FP32 sums over 32 union slots
+                                    moved into FP64 sums, 1 = FP64 sums after every 4 slots
+                                    (about a third more SpMM time; over 5000 steps of the
+                                    12,000-site tube 1.7e-7 instead of 1.6e-6 population
+                                    error).  dq_autotune keeps the value it finds. */
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

@@ -15,11 +15,13 @@ pytestmark = pytest.mark.gpu
 GATHER = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0,
                spmm_window_rows=0) for r in (2, 4, 8) for t in (1, 2)]
 GATHER += [dict(spmm_rows_per_wave=4, spmm_traj_per_lane=1, spmm_phases=2, spmm_pipeline=1, spmm_window_rows=0)]
-MFMA16 = [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=0)
-          for t in (1, 2, 4)]
+MFMA16 = [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=0,
+               spmm_exact_sums=e) for e in (0, 1) for t in (1, 2, 4)]
 # 16-row blocks on 16 x 16 x 4 MFMAs reading the LDS window
 WINDOW = [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=T,
-               spmm_window_waves=w) for T in (240, 128) for t in (1, 2) for w in (4, 16)]
+               spmm_window_waves=w, spmm_exact_sums=0) for T in (240, 128) for t in (1, 2) for w in (4, 16)]
+WINDOW += [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=240,
+                spmm_window_waves=16, spmm_exact_sums=1) for t in (1, 2)]
 
 # 128 trajectories: sizes not multiples of 16, sorted internally by
 # n_active, so the slices see ragged ends
@@ -61,7 +63,8 @@ def test_every_spmm_choice_against_oracle(engine, oracle):
             got[(tuple(sorted(c.items())), t)] = y
     # the 16-row window SpMM runs the MFMA sequence of the 16-row gather SpMM
     for c in WINDOW:
-        m16 = [m for m in MFMA16 if m["spmm_traj_per_lane"] == c["spmm_traj_per_lane"]][0]
+        m16 = [m for m in MFMA16 if m["spmm_traj_per_lane"] == c["spmm_traj_per_lane"]
+               and m["spmm_exact_sums"] == c["spmm_exact_sums"]][0]
         for t in trajs:
             a = got[(tuple(sorted(c.items())), t)]
             b = got[(tuple(sorted(m16.items())), t)]
@@ -74,7 +77,7 @@ def test_window_spmm_is_the_same_for_every_slice_shape(engine):
     sysd = double_tube(engine)
     s = _spmv_system(engine, sysd, 128, None)
     x = np.random.default_rng(3).normal(size=sysd["n"]) * (1 + 0.5j)
-    for group in (WINDOW,):
+    for group in ([c for c in WINDOW if not c["spmm_exact_sums"]], [c for c in WINDOW if c["spmm_exact_sums"]]):
         ref = None
         for c in group:
             s.set_tuning(**c)
@@ -167,4 +170,54 @@ def test_window_choice_rejections_leave_the_system_working(engine):
     assert s.tuning() == before
     s.set_state(np.eye(1, n, 0)[0])
     s.run(5)
+    s.close()
+
+
+def _exact_coupling_matrix(nt=32, seed=5):
+    """n = 3 nt sites: site t < nt couples to nt + 2t with 2^24 + k_t (k_t
+    odd, so its FP32 value is 2^24 + k_t -/+ 1) and to nt + 2t + 1 with -2^24
+    (exact in FP32); the pair sits in one group of 16 union slots"""
+    rng = np.random.default_rng(seed)
+    n = 3 * nt
+    k = 2 * rng.integers(1, 60, nt) + 1
+    big = float(2 ** 24)
+    rows = [[] for _ in range(n)]
+    for t in range(nt):
+        rows[t] += [(nt + 2 * t, big + k[t]), (nt + 2 * t + 1, -big)]
+        rows[nt + 2 * t].append((t, big + k[t]))
+        rows[nt + 2 * t + 1].append((t, -big))
+    rp = np.zeros(n + 1, np.int32)
+    col, val = [], []
+    for i in range(n):
+        for c, v in sorted(rows[i]):
+            col.append(c)
+            val.append(v)
+        rp[i + 1] = len(col)
+    x = np.zeros(n, complex)
+    x[nt:] = 1.0 + 0.5j                                  # x_t = 0 for t < nt
+    return n, rp, np.array(col, np.int32), np.array(val), x, k
+
+
+@pytest.mark.parametrize("choice", GATHER + MFMA16 + WINDOW[:2] + WINDOW[-1:])
+def test_mixed_spmm_uses_exact_couplings(engine, choice):
+    """couplings 2^24 + k_t and -2^24 on x = 1 + i/2: w_t = k_t (1 + i/2)
+    exactly, which FP32 couplings alone (2^24 + k_t -/+ 1) would miss: the
+    row-blocked SpMMs use the FP64 couplings, the matrix-core ones add the
+    low parts of the FP32 couplings (every lane position of their operands:
+    32 rows, unions of two 16-slot groups)"""
+    n, rp, col, val, x, k = _exact_coupling_matrix()
+    nt = n // 3
+    s = engine.System(n, 8, 1.0, n_traj=128, precision=engine.MIXED, sigma_static=0.0, force_batched=True)
+    s.set_hamiltonian(rp, col, val, np.zeros(n))
+    s.set_ou(50.0, 0.0)
+    s.reset_disorder()
+    try:
+        s.set_tuning(**choice)
+    except engine.DQError as e:
+        s.close()
+        pytest.skip("choice not available at n = %d: %s" % (n, e))
+    for traj in (0, 101):
+        y = s.spmv(traj, x)
+        assert np.array_equal(y[:nt], k * (1.0 + 0.5j)), (choice, traj, y[:nt].real - k)
+        assert np.all(y[nt:] == 0)
     s.close()
