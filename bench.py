@@ -77,7 +77,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_N12000_B256_m24.txt")
 def spmm_kernel_name(tuning):
     """the SpMM kernel a tuning launches (mixed precision)"""
     if tuning.get("spmm_window_rows", 0) > 0:
-        return "dq::k_spmm_mfma_win<%d>" % (2 * tuning["spmm_traj_per_lane"])
+        return "dq::k_spmm_ring<%s>" % ("true" if tuning.get("spmm_exact_sums", 0) else "false")
     if tuning["spmm_rows_per_wave"] == 16:
         return "dq::k_spmm_mfma<%d>" % (2 * tuning["spmm_traj_per_lane"])
     return "dq::k_spmm_b<float,%d,%d,%s>" % (tuning["spmm_traj_per_lane"], tuning["spmm_rows_per_wave"],

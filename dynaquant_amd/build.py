@@ -20,7 +20,7 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 
-SOURCES_HIP = ["dq_kernels.hip", "dq_batched.hip", "dq_window.hip", "dq_adjoint.hip", "dq_noalloc.hip"]
+SOURCES_HIP = ["dq_kernels.hip", "dq_batched.hip", "dq_ring.hip", "dq_adjoint.hip", "dq_noalloc.hip"]
 SOURCES_CPP = ["dq_api.cpp", "dq_tube.cpp", "dq_io.cpp"]
 
 

@@ -71,9 +71,9 @@ struct RowBlocks {
     uint16_t* lo16 = nullptr;   // R = 16: bfloat16 low parts val - (double)val32, MFMA operand order
 };
 
-// LDS one workgroup of the window SpMM may use: the device's opt-in limit
+// LDS one workgroup of the ring SpMM may use: the device's opt-in limit
 // (160 KiB on gfx950), queried once; 0 when unknown
-static size_t win_lds_limit()
+static size_t ring_lds_limit()
 {
     static size_t lim = [] {
         int dev = 0, a = 0, b = 0;
@@ -85,12 +85,16 @@ static size_t win_lds_limit()
     return lim;
 }
 
-// Windows of the LDS-window SpMM (dq_window.hip) for tiles of T rows: per
-// tile the first row and the number of rows of every column its rows
-// reference and the rows themselves; the products read the rb = 16 lists.
-struct WinLists {
-    int32_t* lo = nullptr;           // [tile][2] first window row, window rows
-    int T = 0, ts = 0, max_rows = 0, ntile = 0;
+// Lists of the ring SpMM (dq_ring.hip): per 4-row block its column union in
+// even-aligned pairs, in items of up to 16 pairs: the couplings and their
+// bfloat16 low parts in MFMA operand order and the LDS offsets of the pairs;
+// per 128-row chunk the window of x rows it references.
+struct RingLists {
+    int32_t* item = nullptr;         // [ceil(n/4) + 1] first item of each block
+    float* a = nullptr;              // [item][64][2]
+    uint2* alo = nullptr;            // [item][32]
+    uint4* off = nullptr;            // [item][4]
+    int2* win = nullptr;             // [ceil(n/128)] [lo, hi)
 };
 
 struct dq_system {
@@ -102,7 +106,8 @@ struct dq_system {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join[kMaxSlices - 1] = {};
     int slices = 1;                   // captured steps run this many trajectory slices concurrently
-    WinLists win;                     // LDS-window SpMM lists (win.T > 0 when built)
+    RingLists ring;                   // ring SpMM lists (built while ring_rows > 0 and H is set)
+    int ring_rows = 0;                // ring SpMM: rows per workgroup (0: the row-blocked SpMMs)
     // substep control (dq_system_set_krylov_control): chunks of kchunk steps,
     // K doubled up to kmax while K x the largest cycle estimate exceeds ktol
     double hbar = 0.0;
@@ -117,8 +122,8 @@ struct dq_system {
         int* status = nullptr;
         int64_t* step = nullptr;
     } ck;
-    int win_waves = 8;                // waves per workgroup of the window SpMM
-    bool win_lds_set = false;         // dynamic LDS limit raised for the window SpMM
+    int ring_waves = 16;              // waves per workgroup of the ring SpMM
+    bool ring_lds_set = false;        // dynamic LDS limit raised for the ring SpMM
     hipGraphExec_t graph = nullptr;
     double last_ms = 0.0;
     int n = 0, B = 0, Bp = 0, m = 0, K = 1;
@@ -345,7 +350,7 @@ static dq_status make_row_blocks(dq_system* sys, int R, const std::vector<int32_
 
 // The tile and slice counts every SpMM consumer uses: the row-blocked
 // kernels' tiles (4 row blocks of rb rows), and the tiles whose partial sums
-// the SpMM leaves in part_s (the window kernel's when it is on).
+// the SpMM leaves in part_s (the ring kernel's row ranges when it is on).
 static void refresh_spmm_tiles(dq_system* sys)
 {
     Dev& d = sys->d;
@@ -353,7 +358,7 @@ static void refresh_spmm_tiles(dq_system* sys)
         d.rt_s = 4 * d.rb;
         d.ntile_s = (sys->n + d.rt_s - 1) / d.rt_s;
     }
-    d.ntile_p = (d.win_T > 0 && sys->win.lo) ? sys->win.ntile : d.ntile_s;
+    d.ntile_p = d.ring_rows > 0 ? (sys->n + d.ring_rows - 1) / d.ring_rows : d.ntile_s;
 }
 
 // row blocks of R rows for the Hamiltonian (from the host CSR kept by
@@ -385,79 +390,151 @@ static dq_status build_row_blocks(dq_system* sys, int R)
     return DQ_OK;
 }
 
-static void release_window(dq_system* sys, WinLists* w)
+static void release_ring(dq_system* sys, RingLists* r)
 {
-    sys->release(w->lo);
-    *w = WinLists{};
+    sys->release(r->item);
+    sys->release(r->a);
+    sys->release(r->alo);
+    sys->release(r->off);
+    sys->release(r->win);
+    *r = RingLists{};
 }
 
-// Windows of the LDS-window SpMM (dq_window.hip) for tiles of T rows (a
-// multiple of 16: whole 16-row blocks) and slices of ts = 16 or 32
-// trajectories.  Fails (nothing changed) when a window does not fit 65,536
-// rows or one workgroup's LDS.
-static dq_status make_window_lists(dq_system* sys, int T, int ts, WinLists* out)
+// Lists of the ring SpMM for the Hamiltonian's host CSR.  Fails (nothing
+// changed) when the band is too wide: the x rows that one 128-row chunk and
+// the next reference must fit the ring of kRingCap rows.
+static dq_status make_ring_lists(dq_system* sys, RingLists* out)
 {
-    const int n = sys->n;
-    if (ts != 16 && ts != 32) return fail(DQ_ERR_INVALID_ARG, "window SpMM: 16 or 32 trajectories per slice");
-    if (T < 16 || T % 16 != 0 || T > 4096)
-        return fail(DQ_ERR_INVALID_ARG, "window SpMM: tile rows must be a multiple of 16 in [16, 4096]");
+    constexpr int CAP = dq::kRingCap, CH = dq::kRingChunk;
+    const int n = sys->n, nblk = (n + 3) / 4, nchunk = (n + CH - 1) / CH;
     const int32_t* rowptr = sys->rowptr.data();
     const int32_t* col = sys->col.data();
-    const int ntile = (n + T - 1) / T;
-    std::vector<int32_t> lo(2 * (size_t)ntile);
-    int max_rows = 0;
-    for (int t = 0; t < ntile; ++t) {
-        const int r0 = t * T, r1 = std::min(n, r0 + T);
-        int mn = r0, mx = r1 - 1;
+    const double* val = sys->val.data();
+    // windows of the 128-row chunks: their own rows and every column they
+    // reference, made monotone (hi a running maximum, lo a running minimum
+    // from the end) and rounded to groups of 4 rows
+    std::vector<int2> win(nchunk);
+    for (int c = 0; c < nchunk; ++c) {
+        const int r0 = c * CH, r1 = std::min(n, r0 + CH);
+        int lo = r0, hi = r1;
         for (int i = r0; i < r1; ++i)
-            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) { mn = std::min(mn, col[p]); mx = std::max(mx, col[p]); }
-        lo[2 * t] = mn;
-        lo[2 * t + 1] = mx - mn + 1;
-        max_rows = std::max(max_rows, mx - mn + 1);
+            for (int32_t p = rowptr[i]; p < rowptr[i + 1]; ++p) { lo = std::min(lo, col[p]); hi = std::max(hi, col[p] + 1); }
+        win[c] = make_int2(lo, hi);
     }
-    if (max_rows > 65536) return fail(DQ_ERR_INVALID_ARG, "window SpMM: a window exceeds 65,536 rows");
-    const size_t lds = dq::spmm_win_lds_bytes(ts, max_rows), lds_max = win_lds_limit();
-    if (lds > lds_max)
-        return fail(DQ_ERR_INVALID_ARG, "window SpMM: the widest window needs " + std::to_string(lds) +
-                                            " bytes of LDS, more than one workgroup's " + std::to_string(lds_max));
-    WinLists w;
-    DQ_TRY(sys->alloc(&w.lo, lo.size()));
-    hipError_t e = hipMemcpyAsync(w.lo, lo.data(), sizeof(int32_t) * lo.size(), hipMemcpyHostToDevice, sys->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(sys->stream);
-    if (e != hipSuccess) {
-        release_window(sys, &w);
-        return fail(DQ_ERR_HIP, std::string("window SpMM lists: ") + hipGetErrorString(e));
+    const int nr4 = (n + 3) & ~3;
+    for (int c = 1; c < nchunk; ++c) win[c].y = std::max(win[c].y, win[c - 1].y);
+    for (int c = nchunk - 2; c >= 0; --c) win[c].x = std::min(win[c].x, win[c + 1].x);
+    for (int c = 0; c < nchunk; ++c) {
+        win[c].x &= ~3;
+        win[c].y = std::min(nr4, (win[c].y + 3) & ~3);
+        const int span = (c + 1 < nchunk ? win[c + 1].y : win[c].y) - win[c].x;
+        if (span > CAP)
+            return fail(DQ_ERR_INVALID_ARG, "ring SpMM: rows " + std::to_string(c * CH) + ".. reference x rows " +
+                                                std::to_string(span) + " apart, more than its LDS ring of " +
+                                                std::to_string(CAP) + " rows (the Hamiltonian's band is too wide)");
     }
-    if (!sys->win_lds_set) {
-        e = dq::spmm_win_set_lds_limit(lds_max);
+    // per 4-row block: the pairs (2p, 2p + 1) covering its column union
+    // (a block without couplings takes the pair of its own first rows, so
+    // that its item still forms w = d x), padded to a multiple of 4 with
+    // zero-coupling copies of the last pair (rows of its window), in items of
+    // up to 16 pairs
+    std::vector<int32_t> item(nblk + 1, 0);
+    std::vector<float> a;
+    std::vector<uint16_t> alo;
+    std::vector<uint32_t> off;
+    std::vector<int32_t> pr;
+    std::vector<std::pair<int32_t, double>> row[4];
+    int nit = 0;
+    for (int k = 0; k < nblk; ++k) {
+        const int r0 = 4 * k;
+        pr.clear();
+        for (int i = 0; i < 4; ++i) {
+            row[i].clear();
+            if (r0 + i >= n) continue;
+            for (int32_t p = rowptr[r0 + i]; p < rowptr[r0 + i + 1]; ++p) {
+                row[i].push_back({col[p], val[p]});
+                pr.push_back(col[p] >> 1);
+            }
+            std::sort(row[i].begin(), row[i].end());
+        }
+        std::sort(pr.begin(), pr.end());
+        pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+        if (pr.empty()) pr.push_back(r0 >> 1);
+        const int np = (int)pr.size(), np4 = (np + 3) & ~3, nk = (np4 + 15) / 16;
+        item[k] = nit;
+        a.resize((size_t)(nit + nk) * 128, 0.f);
+        alo.resize((size_t)(nit + nk) * 128, 0);
+        off.resize((size_t)(nit + nk) * 16, 0);
+        for (int q = 0; q < 16 * nk; ++q) {
+            const int it = nit + q / 16, sl = q % 16;
+            const int pair = q < np ? pr[q] : pr[np - 1];
+            off[(size_t)it * 16 + sl] = (uint32_t)(((2 * pair) & (CAP - 1)) * 4);
+            if (q >= np) continue;
+            for (int h = 0; h < 2; ++h)
+                for (int i = 0; i < 4; ++i) {
+                    const auto e = std::lower_bound(row[i].begin(), row[i].end(), std::make_pair(2 * pair + h, -1e308));
+                    const double v = (e != row[i].end() && e->first == 2 * pair + h) ? e->second : 0.0;
+                    const float v32 = (float)v;
+                    a[((size_t)it * 64 + 32 * h + 4 * (sl % 8) + i) * 2 + sl / 8] = v32;
+                    alo[(size_t)it * 128 + (((sl / 4) * 2 + h) * 4 + i) * 4 + sl % 4] = bf16_bits((float)(v - (double)v32));
+                }
+        }
+        // the item's pair count and the block's last item in the low bits of
+        // its first offset (a multiple of 8)
+        for (int t = 0; t < nk; ++t) {
+            const int cnt = std::min(16, np4 - 16 * t);
+            off[(size_t)(nit + t) * 16] |= (uint32_t)(cnt / 4 - 1) | (t == nk - 1 ? 4u : 0u);
+        }
+        nit += nk;
+    }
+    item[nblk] = nit;
+    if ((uint64_t)nit * 512 >= ((uint64_t)1 << 31))
+        return fail(DQ_ERR_INVALID_ARG, "ring SpMM: the coupling items exceed 2^31 bytes");
+    Pending pend(sys);
+    RingLists r;
+    DQ_TRY(pend.alloc(&r.item, (size_t)nblk + 1));
+    DQ_TRY(pend.alloc(&r.a, a.size()));
+    DQ_TRY(pend.alloc(&r.alo, alo.size() / 4));
+    DQ_TRY(pend.alloc(&r.off, off.size() / 4));
+    DQ_TRY(pend.alloc(&r.win, (size_t)nchunk));
+    HIP_TRY(hipMemcpyAsync(r.item, item.data(), sizeof(int32_t) * item.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(r.a, a.data(), sizeof(float) * a.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(r.alo, alo.data(), sizeof(uint16_t) * alo.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(r.off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipMemcpyAsync(r.win, win.data(), sizeof(int2) * win.size(), hipMemcpyHostToDevice, sys->stream));
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    if (!sys->ring_lds_set) {
+        const size_t need = dq::spmm_ring_lds_bytes(16, dq::kRingMaxRows), lim = ring_lds_limit();
+        if (need > lim)
+            return fail(DQ_ERR_INVALID_ARG, "ring SpMM: needs " + std::to_string(need) + " bytes of LDS per "
+                                                "workgroup, the device allows " + std::to_string(lim));
+        const hipError_t e = dq::spmm_ring_set_lds_limit();
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            release_window(sys, &w);
-            return fail(DQ_ERR_INVALID_ARG, std::string("window SpMM: the LDS limit cannot be raised: ") +
+            return fail(DQ_ERR_INVALID_ARG, std::string("ring SpMM: the LDS limit cannot be raised: ") +
                                                 hipGetErrorString(e));
         }
-        sys->win_lds_set = true;
+        sys->ring_lds_set = true;
     }
-    w.T = T;
-    w.ts = ts;
-    w.max_rows = max_rows;
-    w.ntile = ntile;
-    release_window(sys, out);
-    *out = w;
+    pend.keep();
+    release_ring(sys, out);
+    *out = r;
     return DQ_OK;
 }
 
-// Point the device view at the window lists (T > 0) or switch the window
-// SpMM off (T = 0).
-static void attach_window(dq_system* sys)
+// Point the device view at the ring lists (ring_rows > 0 and lists built) or
+// switch the ring SpMM off.
+static void attach_ring(dq_system* sys)
 {
     Dev& d = sys->d;
-    const WinLists& w = sys->win;
-    d.win_T = w.lo ? w.T : 0;
-    d.win_ts = w.ts;
-    d.win_w = sys->win_waves;
-    d.win_lds = d.win_T > 0 ? (int)dq::spmm_win_lds_bytes(w.ts, w.max_rows) : 0;
-    d.win_lo = w.lo;
+    const RingLists& r = sys->ring;
+    d.ring_rows = (sys->ring_rows > 0 && r.item) ? sys->ring_rows : 0;
+    d.ring_w = sys->ring_waves;
+    d.ring_item = r.item;
+    d.ring_a = r.a;
+    d.ring_alo = r.alo;
+    d.ring_off = r.off;
+    d.ring_win = r.win;
     refresh_spmm_tiles(sys);
 }
 
@@ -515,7 +592,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.ntile_p = d.ntile_s;
-    d.win_w = s->win_waves;
+    d.ring_w = s->ring_waves;
     d.S = cfg->ou_substeps;
     d.noise_model = cfg->noise_model;
     d.dt = cfg->dt_fs;
@@ -625,7 +702,7 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
     Dev& d = sys->d;
     sys->drop_graph();
     HIP_TRY(hipStreamSynchronize(sys->stream));
-    // the new CSR, row blocks and window lists are complete before anything
+    // the new CSR, row blocks and ring lists are complete before anything
     // old is released: a failure leaves the previous Hamiltonian in place
     Pending pend(sys);
     int32_t *rp, *cl;
@@ -659,22 +736,22 @@ dq_status dq_system_set_hamiltonian_csr(dq_system* sys, int32_t n, const int32_t
         std::swap(sys->col, col_h);
         std::swap(sys->val, val_h);
     };
-    WinLists nw;
-    if (sys->win.T > 0) {
-        dq_status st = make_window_lists(sys, sys->win.T, sys->win.ts, &nw);
+    RingLists nr;
+    if (sys->ring_rows > 0) {
+        dq_status st = make_ring_lists(sys, &nr);
         if (st != DQ_OK) { restore_host(); return st; }
     }
     dq_status st = build_row_blocks(sys, d.rb);
     if (st != DQ_OK) {
-        release_window(sys, &nw);
+        release_ring(sys, &nr);
         restore_host();
         return st;
     }
     pend.keep();
-    if (sys->win.T > 0) {
-        release_window(sys, &sys->win);
-        sys->win = nw;
-        attach_window(sys);
+    if (sys->ring_rows > 0) {
+        release_ring(sys, &sys->ring);
+        sys->ring = nr;
+        attach_ring(sys);
     }
     sys->release(*(int32_t**)&d.rowptr);
     sys->release(*(int32_t**)&d.col);
@@ -1249,7 +1326,7 @@ static Dev tk_dev(const dq_system* sys, const Dev& d)
     a.diag = sys->tk.diag;
     a.g = 1.0;
     // the auxiliary matrix goes through the row-blocked SpMM
-    a.win_T = 0;
+    a.ring_rows = 0;
     a.ntile_p = a.ntile_s;
     return a;
 }
@@ -2119,7 +2196,7 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
         *why = "spmm_traj_per_lane must be 1 or 2";
         return false;
     }
-    if (t.spmm_rows_per_wave != 16 && t.spmm_window_rows == 0 && t.spmm_traj_per_lane == 2 &&
+    if (t.spmm_rows_per_wave != 16 && t.spmm_traj_per_lane == 2 &&
         !(d.mixed && d.Bp % 128 == 0)) {
         *why = "spmm_traj_per_lane = 2 needs mixed precision and a padded batch that is a multiple of 128";
         return false;
@@ -2156,22 +2233,21 @@ static bool tuning_valid(const dq_system* sys, const dq_tuning& t, std::string* 
     }
     if (t.spmm_window_rows > 0) {
         if (!d.mixed) {
-            *why = "the LDS-window SpMM (spmm_window_rows > 0) is for mixed precision";
+            *why = "the ring SpMM (spmm_window_rows > 0) is for mixed precision";
             return false;
         }
-        if (t.spmm_rows_per_wave != 16 || (t.spmm_traj_per_lane != 1 && t.spmm_traj_per_lane != 2) ||
-            t.spmm_phases != 1 || t.spmm_pipeline != 0) {
-            *why = "the LDS-window SpMM needs spmm_rows_per_wave = 16, spmm_traj_per_lane 1 or 2 (16 or 32 "
-                   "trajectories per slice), spmm_phases = 1 and spmm_pipeline = 0";
+        if (t.spmm_window_rows % dq::kRingChunk != 0 || t.spmm_window_rows > dq::kRingMaxRows) {
+            *why = "spmm_window_rows must be a multiple of 128, at most 4096";
             return false;
         }
-        if (t.spmm_window_rows % 16 != 0 || t.spmm_window_rows > 4096) {
-            *why = "spmm_window_rows must be a multiple of 16, at most 4096";
+        if ((uint64_t)sys->n * (uint64_t)d.Bp * 8 >= ((uint64_t)1 << 31)) {
+            *why = "the ring SpMM addresses a Krylov vector with 32-bit offsets: n * padded batch * 8 "
+                   "must stay below 2^31 bytes";
             return false;
         }
         if (t.spmm_window_waves != 0 && t.spmm_window_waves != 4 && t.spmm_window_waves != 8 &&
             t.spmm_window_waves != 16) {
-            *why = "spmm_window_waves must be 4, 8 or 16 (0: 8)";
+            *why = "spmm_window_waves must be 4, 8 or 16 (0: 16)";
             return false;
         }
     }
@@ -2188,43 +2264,37 @@ static void get_tuning(const dq_system* sys, dq_tuning* t)
     t->spmm_pipeline = sys->d.spmm_pipe;
     t->spmm_exact_sums = sys->d.spmm_exact;
     t->concurrent_slices = sys->slices;
-    t->spmm_window_rows = sys->win.T;
-    t->spmm_window_waves = sys->win.T > 0 ? sys->win_waves : 0;
+    t->spmm_window_rows = sys->ring_rows;
+    t->spmm_window_waves = sys->ring_rows > 0 ? sys->ring_waves : 0;
 }
 
-// Applies a validated tuning; the row blocks and window lists it needs are
-// built before anything is replaced, so a failure (a window wider than one
-// workgroup's LDS) leaves the previous choice in place.
+// Applies a validated tuning; the row blocks and ring lists it needs are
+// built before anything is replaced, so a failure (a band too wide for the
+// ring) leaves the previous choice in place.
 static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
 {
     Dev& d = sys->d;
-    const int T = t.spmm_window_rows, ts = 16 * t.spmm_traj_per_lane;
-    WinLists nw;
-    const bool new_win = T > 0 && sys->have_h && (sys->win.T != T || sys->win.ts != ts || !sys->win.lo);
-    if (new_win) DQ_TRY(make_window_lists(sys, T, ts, &nw));
+    const int T = t.spmm_window_rows;
+    RingLists nr;
+    const bool new_ring = T > 0 && sys->have_h && !sys->ring.item;
+    if (new_ring) DQ_TRY(make_ring_lists(sys, &nr));
     if (t.spmm_rows_per_wave != d.rb) {
         if (sys->have_h) {
             dq_status st = build_row_blocks(sys, t.spmm_rows_per_wave);
             if (st != DQ_OK) {
-                release_window(sys, &nw);
+                release_ring(sys, &nr);
                 return st;
             }
         } else {
             d.rb = t.spmm_rows_per_wave;
         }
     }
-    if (T == 0) {
-        release_window(sys, &sys->win);
-    } else if (new_win) {
-        release_window(sys, &sys->win);
-        sys->win = nw;
-    } else if (!sys->have_h) {
-        // built by set_hamiltonian
-        release_window(sys, &sys->win);
-        sys->win.T = T;
-        sys->win.ts = ts;
-    }
-    sys->win_waves = t.spmm_window_waves > 0 ? t.spmm_window_waves : 8;
+    // the lists follow the Hamiltonian only while the ring SpMM is chosen
+    // (set_hamiltonian rebuilds them then), so they go when it is not
+    if (T == 0) release_ring(sys, &sys->ring);
+    else if (new_ring) sys->ring = nr;
+    sys->ring_rows = T;
+    sys->ring_waves = t.spmm_window_waves > 0 ? t.spmm_window_waves : 16;
     d.spmm_tpl = t.spmm_traj_per_lane;
     d.spmm_phases = t.spmm_phases;
     d.spmm_pipe = t.spmm_pipeline;
@@ -2232,7 +2302,7 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
     sys->slices = t.concurrent_slices < 1 ? 1 : t.concurrent_slices;
     d.rt_e = t.tile_rows;
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
-    attach_window(sys);
+    attach_ring(sys);
     return DQ_OK;
 }
 
@@ -2384,8 +2454,8 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     hipError_t e = hipMemcpyAsync(saved, d.psi, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(saved_de, d.de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(saved_diag, d.diag, diag_bytes, hipMemcpyDeviceToDevice, sys->stream);
-    // candidates: the row-blocked SpMMs, then (mixed precision) the
-    // LDS-window SpMM, grouped so that each list is built once; the caller's
+    // candidates: the row-blocked SpMMs, then (mixed precision) the ring
+    // SpMM, grouped so that each list is built once; the caller's
     // spmm_exact_sums is an accuracy choice and is kept
     const int32_t exact = orig.spmm_exact_sums;
     std::vector<dq_tuning> cands;
@@ -2408,27 +2478,26 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                     cands.push_back(t);
                 }
     if (d.mixed)
-        for (int T : {240, 128, 64, 256})
-            for (int tpl : {2, 1})
-                for (int W : {8, 4, 16})
-                    for (int cand = 0; cand < 4; ++cand) {
-                        dq_tuning t;
-                        std::memset(&t, 0, sizeof(t));
-                        t.spmm_rows_per_wave = 16;
-                        t.spmm_traj_per_lane = tpl;
-                        t.spmm_phases = 1;
-                        t.tile_rows = rows[cand];
-                        t.spmm_window_rows = T;
-                        t.spmm_window_waves = W;
-                        t.spmm_exact_sums = exact;
-                        cands.push_back(t);
-                    }
+        for (int T : {768, 384, 1536})
+            for (int W : {16, 8})
+                for (int cand = 0; cand < 4; ++cand) {
+                    dq_tuning t;
+                    std::memset(&t, 0, sizeof(t));
+                    t.spmm_rows_per_wave = 4;
+                    t.spmm_traj_per_lane = d.Bp % 128 == 0 ? 2 : 1;
+                    t.spmm_phases = 1;
+                    t.tile_rows = rows[cand];
+                    t.spmm_window_rows = T;
+                    t.spmm_window_waves = W;
+                    t.spmm_exact_sums = exact;
+                    cands.push_back(t);
+                }
     for (const dq_tuning& t : cands) {
         if (e != hipSuccess || st != DQ_OK) break;
         std::string why;
         if (!tuning_valid(sys, t, &why)) continue;
         const dq_status sa = apply_tuning(sys, t);
-        if (sa == DQ_ERR_INVALID_ARG && t.spmm_window_rows > 0) continue;   // window wider than the LDS
+        if (sa == DQ_ERR_INVALID_ARG && t.spmm_window_rows > 0) continue;   // band too wide for the ring
         if ((st = sa) != DQ_OK) break;
         if ((e = enqueue_tuning_cycle(sys)) != hipSuccess) break;       // warm-up
         float ms_min = 1e30f;

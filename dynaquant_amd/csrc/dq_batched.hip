@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
 
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
 {
-    if (d.win_T > 0) return launch_spmm_win(d, j, last, s);
+    if (d.ring_rows > 0) return launch_spmm_ring(d, j, last, s);
     if (d.rb == 16) {
         // the matrix-core kernel (mixed precision only, enforced by the tuning
         // checks); spmm_traj_per_lane 1, 2, 4 = 16, 32, 64 trajectories per wave

@@ -16,6 +16,9 @@ constexpr int kMaxGroups = 8;      // population groups reduced on the device
 constexpr int kThreads = 256;
 constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
 constexpr int kRowsMin = 32;       // smallest elementwise tile (rows per workgroup)
+constexpr int kRingCap = 1024;     // ring SpMM: x rows held per column in LDS (a power of 2)
+constexpr int kRingChunk = 128;    // ring SpMM: rows multiplied between two ring refills
+constexpr int kRingMaxRows = 4096; // ring SpMM: most rows per workgroup
 
 struct Dev {
     int n, Bp, B, m;
@@ -24,7 +27,7 @@ struct Dev {
     int rt_e, ntile_e;              // elementwise kernels: rows per tile, tiles
     int rt_s, ntile_s;              // SpMM kernels
     int ntile_p;                    // tiles of the SpMM partials in part_s (ntile_s, or the
-                                    // window kernel's tiles when win_T > 0)
+                                    // ring kernel's row ranges when ring_rows > 0)
     int rb;                         // batched SpMM: rows per wavefront (row block)
     int spmm_tpl;                   // batched FP32 SpMM: trajectories per lane (1 or 2)
     int spmm_phases;                // batched SpMM: trajectory slices run as this many launches
@@ -55,10 +58,17 @@ struct Dev {
     const float* rb_val32;          // [total][rb]
     const double* rb_val64;         // [total][rb]
     const uint16_t* rb_lo16;        // rb = 16: bfloat16 low parts of rb_val32 (16-slot MFMA operands)
-    // LDS-window SpMM (dq_window.hip), mixed precision: tiles of win_T rows,
-    // slices of win_ts trajectories, win_w waves per workgroup
-    int win_T, win_ts, win_w, win_lds;
-    const int32_t* win_lo;          // [tile][2] first window row, window rows
+    // ring SpMM (dq_ring.hip), mixed precision: a workgroup walks ring_rows
+    // rows (a multiple of kRingChunk; 0 = off) for 16 trajectories with
+    // ring_w waves; lists built from the CSR for 4-row blocks
+    int ring_rows, ring_w;
+    const int32_t* ring_item;       // [ceil(n/4) + 1] first item (up to 16 column pairs) of each 4-row block
+    const float* ring_a;            // [item][64][2] FP32 couplings: lane 32h + 4a + i, row i, column
+                                    // 2 p + h of pair a (.x) and pair 8 + a (.y) of the item
+    const uint2* ring_alo;          // [item][group 4][h][row 4] bfloat16 low parts of the 4 pairs
+    const uint4* ring_off;          // [item][4] LDS byte offsets (2 p mod kRingCap) * 4 of the 16 pairs;
+                                    // bits 0-2 of the first: pairs / 4 - 1, last item of the block
+    const int2* ring_win;           // [ceil(n/kRingChunk)] window rows [lo, hi) of each 128-row chunk
     const double* inv_tau;          // [n]
     const double* coeff;            // [n] sigma*sqrt(2/tau)
     const double* ou_sigma;         // [n]
@@ -139,10 +149,10 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
 // (part, 2j+1 quantities) into red[0..2j], in one launch
 hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
-// LDS-window SpMM (dq_window.hip)
-hipError_t launch_spmm_win(const Dev& d, int j, int last, hipStream_t s);
-size_t spmm_win_lds_bytes(int ts, int max_rows);
-hipError_t spmm_win_set_lds_limit(size_t bytes);
+// ring SpMM (dq_ring.hip)
+hipError_t launch_spmm_ring(const Dev& d, int j, int last, hipStream_t s);
+size_t spmm_ring_lds_bytes(int waves, int rows);
+hipError_t spmm_ring_set_lds_limit();
 hipError_t launch_combine_b(const Dev& d, hipStream_t s);
 hipError_t launch_pade(const Dev& d, hipStream_t s);
 hipError_t launch_combine(const Dev& d, hipStream_t s);

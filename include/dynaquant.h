@@ -345,24 +345,25 @@ typedef struct dq_tuning {
     int32_t concurrent_slices;   /* 1, 2 or 4: a graph-captured step runs that many equal slices
                                     of the trajectory slots on their own streams (each slice a
                                     multiple of 64 slots) */
-    int32_t spmm_window_rows;    /* 0: the SpMMs above.  T > 0 (a multiple of 16, <= 4096):
-                                    the LDS-window SpMM (mixed precision only): a workgroup
-                                    stages the x rows a tile of T rows references in LDS and
-                                    multiplies 16-row blocks on the matrix cores, bit for bit
-                                    the result of spmm_rows_per_wave = 16; needs
-                                    spmm_rows_per_wave = 16, spmm_phases = 1, spmm_pipeline = 0,
-                                    spmm_traj_per_lane 1 or 2 (16 or 32 trajectories per wave)
-                                    and every tile's window (8 * 16 * spmm_traj_per_lane bytes
-                                    per referenced row, a banded matrix keeps it short) within
-                                    the device's LDS per workgroup (160 KiB on gfx950), else
-                                    DQ_ERR_INVALID_ARG */
-    int32_t spmm_window_waves;   /* 0 (8), 4, 8 or 16 waves per workgroup of the LDS-window SpMM */
+    int32_t spmm_window_rows;    /* 0: the SpMMs above.  T > 0 (a multiple of 128, <= 4096): the ring
+                                    SpMM (mixed precision only): a workgroup walks T rows for
+                                    16 trajectories, holding the x rows they reference in an
+                                    LDS ring of 1024 rows and multiplying 4-row blocks on the
+                                    matrix cores; spmm_rows_per_wave and spmm_traj_per_lane
+                                    then only shape the other kernels.  A Hamiltonian whose
+                                    rows r .. r + 255 reference x rows more than 1024 apart
+                                    (after rounding to groups of 4) is refused with
+                                    DQ_ERR_INVALID_ARG, by dq_set_tuning or, with the ring
+                                    chosen, by dq_system_set_hamiltonian */
+    int32_t spmm_window_waves;   /* 0 (16), 4, 8 or 16 waves per workgroup of the ring SpMM */
     int32_t spmm_exact_sums;     /* matrix-core SpMMs (mixed precision; the row-blocked ones
                                     always sum in FP64): 0 = FP32 sums over 32 union slots
                                     moved into FP64 sums, 1 = FP64 sums after every 4 slots
                                     (about a third more SpMM time; over 5000 steps of the
                                     12,000-site tube 1.7e-7 instead of 1.6e-6 population
-                                    error).  dq_autotune keeps the value it finds. */
+                                    error).  Ring SpMM: FP32 sums of 8 (0) or 4 (1) products
+                                    per output moved into FP64 sums.  dq_autotune keeps the
+                                    value it finds. */
 } dq_tuning;
 
 dq_status dq_get_tuning(dq_system *sys, dq_tuning *out);

@@ -1,7 +1,8 @@
 """Batched mixed-precision SpMM, every kernel choice, against the oracle's
-SpMV on the 12,000-site double tube with ragged trajectory sizes; the
-LDS-window SpMM (16-row blocks on the matrix cores, x rows staged in LDS)
-bitwise against the 16-row gather SpMM and through whole propagations."""
+SpMV on the 12,000-site double tube with ragged trajectory sizes; the ring
+SpMM (x rows staged in an LDS ring, 4-row blocks on the matrix cores) the
+same bit for bit for every row range and wave count, and through whole
+propagations against the FP64 path."""
 import numpy as np
 import pytest
 
@@ -10,18 +11,17 @@ from tests.helpers import double_tube, make_engine, outer_wall_state, ring_tube
 pytestmark = pytest.mark.gpu
 
 # row-blocked gathers (R = 2, 4, 8; 64 or 128 trajectories per wave), the
-# 16-row matrix-core SpMM (16, 32, 64 trajectories per wave) and the
-# LDS-window SpMM (tiles of 120 / 240 rows, 16 / 32 trajectories per wave)
+# 16-row matrix-core SpMM (16, 32, 64 trajectories per wave) and the ring
+# SpMM (ranges of 128 .. 1536 rows per workgroup, 4 .. 16 waves)
 GATHER = [dict(spmm_rows_per_wave=r, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0,
                spmm_window_rows=0) for r in (2, 4, 8) for t in (1, 2)]
 GATHER += [dict(spmm_rows_per_wave=4, spmm_traj_per_lane=1, spmm_phases=2, spmm_pipeline=1, spmm_window_rows=0)]
 MFMA16 = [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=0,
                spmm_exact_sums=e) for e in (0, 1) for t in (1, 2, 4)]
-# 16-row blocks on 16 x 16 x 4 MFMAs reading the LDS window
-WINDOW = [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=T,
-               spmm_window_waves=w, spmm_exact_sums=0) for T in (240, 128) for t in (1, 2) for w in (4, 16)]
-WINDOW += [dict(spmm_rows_per_wave=16, spmm_traj_per_lane=t, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=240,
-                spmm_window_waves=16, spmm_exact_sums=1) for t in (1, 2)]
+# 4-row blocks on 4 x 4 x 1 MFMAs reading the LDS ring
+RING = [dict(spmm_rows_per_wave=4, spmm_traj_per_lane=1, spmm_phases=1, spmm_pipeline=0, spmm_window_rows=T,
+             spmm_window_waves=w, spmm_exact_sums=e) for e in (0, 1) for T, w in ((768, 16), (384, 8), (128, 4),
+                                                                                   (1536, 16))]
 
 # 128 trajectories: sizes not multiples of 16, sorted internally by
 # n_active, so the slices see ragged ends
@@ -52,7 +52,7 @@ def test_every_spmm_choice_against_oracle(engine, oracle):
     trajs = [0, 1, 2, 3, 4, 5, 9, 58]
     refs = {t: _oracle_spmv(oracle, sysd, d[t], x32, N_ACTIVE[t]) for t in trajs}
     got = {}
-    for c in GATHER + MFMA16 + WINDOW:
+    for c in GATHER + MFMA16 + RING:
         s.set_tuning(**c)
         for t in trajs:
             y = s.spmv(t, x)
@@ -61,27 +61,28 @@ def test_every_spmm_choice_against_oracle(engine, oracle):
             assert err < 2e-6, (c, t, err)
             assert np.all(y[N_ACTIVE[t]:] == 0), (c, t)
             got[(tuple(sorted(c.items())), t)] = y
-    # the 16-row window SpMM runs the MFMA sequence of the 16-row gather SpMM
-    for c in WINDOW:
-        m16 = [m for m in MFMA16 if m["spmm_traj_per_lane"] == c["spmm_traj_per_lane"]
-               and m["spmm_exact_sums"] == c["spmm_exact_sums"]][0]
+    # a block's products do not depend on the row range or the wave count
+    for e in (0, 1):
+        group = [c for c in RING if c["spmm_exact_sums"] == e]
         for t in trajs:
-            a = got[(tuple(sorted(c.items())), t)]
-            b = got[(tuple(sorted(m16.items())), t)]
-            assert np.array_equal(a, b), (c, t, np.abs(a - b).max())
+            a = got[(tuple(sorted(group[0].items())), t)]
+            for c in group[1:]:
+                b = got[(tuple(sorted(c.items())), t)]
+                assert np.array_equal(a, b), (c, t, np.abs(a - b).max())
     s.close()
 
 
-def test_window_spmm_is_the_same_for_every_slice_shape(engine):
-    """the 16- and 32-trajectory slices and the wave counts change no bit of H x"""
+def test_ring_spmm_is_the_same_for_every_range(engine):
+    """row ranges and wave counts change no bit of H x (every trajectory slot
+    position of a 128-slot batch)"""
     sysd = double_tube(engine)
     s = _spmv_system(engine, sysd, 128, None)
     x = np.random.default_rng(3).normal(size=sysd["n"]) * (1 + 0.5j)
-    for group in ([c for c in WINDOW if not c["spmm_exact_sums"]], [c for c in WINDOW if c["spmm_exact_sums"]]):
+    for group in ([c for c in RING if not c["spmm_exact_sums"]], [c for c in RING if c["spmm_exact_sums"]]):
         ref = None
         for c in group:
             s.set_tuning(**c)
-            y = s.spmv(5, x)
+            y = np.stack([s.spmv(t, x) for t in (5, 16, 127)])
             if ref is None:
                 ref = y
             assert np.array_equal(y, ref), c
@@ -102,13 +103,13 @@ def _propagate(engine, sysd, tuning, precision, B=64, steps=200, graph=True, n_a
     return out
 
 
-def test_window_propagation_against_fp64(engine):
-    """600-site ring tube, 200 steps: the window SpMM in the mixed path stays
+def test_ring_propagation_against_fp64(engine):
+    """600-site ring tube, 200 steps: the ring SpMM in the mixed path stays
     within the FP32 budget of the FP64 path and of the VALU SpMM's mixed run"""
     sysd = ring_tube(engine)
     p64, _, _ = _propagate(engine, sysd, None, engine.FP64)
     pv, _, _ = _propagate(engine, sysd, GATHER[2], engine.MIXED)
-    for c in (WINDOW[1], WINDOW[6]):
+    for c in (RING[0], RING[2], RING[5]):
         pw, psiw, errw = _propagate(engine, sysd, c, engine.MIXED)
         e64 = np.abs(pw - p64).max()
         ev = np.abs(pw - pv).max()
@@ -117,12 +118,12 @@ def test_window_propagation_against_fp64(engine):
         assert ev < 1e-6
 
 
-def test_window_ragged_graph_and_slices_bitwise(engine):
+def test_ring_ragged_graph_and_slices_bitwise(engine):
     """ragged sizes, four concurrent slices in the captured step: bitwise the
     eager one-stream run"""
     sysd = ring_tube(engine)
     na = np.array([600, 599, 421, 300, 61, 600, 517, 8] * 32, np.int32)
-    for c in (WINDOW[2], WINDOW[5]):
+    for c in (RING[2], RING[4]):
         a = _propagate(engine, sysd, c, engine.MIXED, B=256, steps=60, graph=False, n_active=na)
         b = _propagate(engine, sysd, c, engine.MIXED, B=256, steps=60, graph=True, n_active=na, slices=4)
         for u, v in zip(a, b):
@@ -132,21 +133,21 @@ def test_window_ragged_graph_and_slices_bitwise(engine):
             assert abs(pops[t, 0].sum() - 1.0) < 1e-5
 
 
-def test_window_choice_rejections_leave_the_system_working(engine):
+def test_ring_choice_rejections_leave_the_system_working(engine):
     sysd = ring_tube(engine)
     s = make_engine(engine, sysd, 16, 1.0, B=64, precision=engine.FP64, force_batched=True)
     with pytest.raises(engine.DQError):
-        s.set_tuning(**WINDOW[0])                       # FP64
+        s.set_tuning(**RING[0])                         # FP64
     s.close()
     s = make_engine(engine, sysd, 16, 1.0, B=64, precision=engine.MIXED)
     with pytest.raises(engine.DQError):
-        s.set_tuning(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_window_rows=120, spmm_window_waves=8)
+        s.set_tuning(spmm_rows_per_wave=4, spmm_window_rows=120, spmm_window_waves=8)
     with pytest.raises(engine.DQError):
-        s.set_tuning(spmm_rows_per_wave=4, spmm_traj_per_lane=2, spmm_window_rows=128, spmm_window_waves=8)
-    s.set_tuning(**WINDOW[1])
+        s.set_tuning(spmm_rows_per_wave=4, spmm_window_rows=128, spmm_window_waves=2)
+    s.set_tuning(**RING[1])
     before = s.tuning()
     # a coupling between the first and the last site of the 12,000-site tube
-    # widens the windows past the LDS of a workgroup
+    # widens the band past the LDS ring of 1024 rows
     big = double_tube(engine)
     n2 = big["n"]
     rows = [list(zip(big["col"][big["rp"][i]:big["rp"][i + 1]], big["val"][big["rp"][i]:big["rp"][i + 1]]))
@@ -161,7 +162,7 @@ def test_window_choice_rejections_leave_the_system_working(engine):
         v2 += [float(b) for _, b in r]
         rp2[i + 1] = len(c2)
     wide = engine.System(n2, 16, 1.0, n_traj=64, precision=engine.MIXED)
-    wide.set_tuning(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_window_rows=2048, spmm_window_waves=8)
+    wide.set_tuning(spmm_rows_per_wave=4, spmm_window_rows=768, spmm_window_waves=16)
     with pytest.raises(engine.DQError):
         wide.set_hamiltonian(rp2, np.array(c2, np.int32), np.array(v2), np.zeros(n2))
     wide.close()
@@ -198,13 +199,14 @@ def _exact_coupling_matrix(nt=32, seed=5):
     return n, rp, np.array(col, np.int32), np.array(val), x, k
 
 
-@pytest.mark.parametrize("choice", GATHER + MFMA16 + WINDOW[:2] + WINDOW[-1:])
+@pytest.mark.parametrize("choice", GATHER + MFMA16 + RING[:1] + RING[-1:])
 def test_mixed_spmm_uses_exact_couplings(engine, choice):
     """couplings 2^24 + k_t and -2^24 on x = 1 + i/2: w_t = k_t (1 + i/2)
     exactly, which FP32 couplings alone (2^24 + k_t -/+ 1) would miss: the
     row-blocked SpMMs use the FP64 couplings, the matrix-core ones add the
     low parts of the FP32 couplings (every lane position of their operands:
-    32 rows, unions of two 16-slot groups)"""
+    32 rows, unions of two 16-slot groups; the ring SpMM multiplies the two
+    columns of a pair in different lane halves, which sum apart)"""
     n, rp, col, val, x, k = _exact_coupling_matrix()
     nt = n // 3
     s = engine.System(n, 8, 1.0, n_traj=128, precision=engine.MIXED, sigma_static=0.0, force_batched=True)
