@@ -866,13 +866,19 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
 __global__ __launch_bounds__(1024) void k_reduce_all(Dev d, int j)
 {
     const int y = blockIdx.y;
-    const int b = d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
+    const int b0 = d.s0 + blockIdx.x * 64, b = b0 + (threadIdx.x & 63);
     const int Bp = d.Bp;
-    if (y < 2)
-        wide_tile_sum(d.part_s + (size_t)y * d.ntile_p * Bp + b, d.ntile_p, Bp,
+    // tiles past the largest n_active of the 64 slots (the first, slots are
+    // sorted) hold exact zeros: summing only the others gives the same bits
+    const int maxn = min(d.n, (int)uniform(d.nact[b0]));
+    if (y < 2) {
+        const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
+        wide_tile_sum(d.part_s + (size_t)y * d.ntile_p * Bp + b, min(d.ntile_p, (maxn + th - 1) / th), Bp,
                       d.red + ((size_t)kRedC(d.m) + y) * Bp + b);
-    else
-        wide_tile_sum(d.part + (size_t)(y - 2) * d.ntile_e * Bp + b, d.ntile_e, Bp, d.red + (size_t)(y - 2) * Bp + b);
+    } else {
+        wide_tile_sum(d.part + (size_t)(y - 2) * d.ntile_e * Bp + b, min(d.ntile_e, (maxn + d.rt_e - 1) / d.rt_e),
+                      Bp, d.red + (size_t)(y - 2) * Bp + b);
+    }
 }
 
 hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s)
