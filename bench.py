@@ -90,7 +90,7 @@ def spmm_kernel_name(tuning):
 
 
 def update_kernel_name(tuning):
-    return "dq::k_update_b<float,KW> tile_rows=%d" % tuning["tile_rows"]
+    return "dq::k_update_b<float,KW,NT> tile_rows=%d" % tuning["tile_rows"]
 
 
 def measured_traffic(kernel):
@@ -310,7 +310,8 @@ def main():
     upd_name, spmm_name = update_kernel_name(tuning), spmm_kernel_name(tuning)
     upd_roof = {"bound": "hbm", "achieved": upd_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(upd_name),
-                "kernel": upd_name + " (KW = ceil((j+1)/4) = 1..6 for j = 0..23)",
+                "kernel": upd_name + " (KW = ceil((j+1)/4) = 1..6 for j = 0..23; the NT = max(0, KW - 2)"
+                          " oldest groups of 4 vectors read with the nontemporal hint)",
                 "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms}
     spmm_roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(spmm_name),

@@ -616,6 +616,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
 #define ALLOC(p, cnt) do { if ((st = s->alloc(p, cnt)) != DQ_OK) { delete s; return st; } } while (0)
     ALLOC(&psi, nB);
     d.vs = (nB + 63) / 64 * 64 + 32;
+    d.upd_cached = (int)std::min<size_t>(dq::kMaxM / 4, dq::kUpdCacheBytes / (4 * nB * (d.mixed ? 8 : 16)));
     if (d.mixed) { float2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; float* dg; ALLOC(&dg, nB); diag = dg; }
     else { double2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; double* dg; ALLOC(&dg, nB); diag = dg; }
     d.psi = psi; d.V = V; d.diag = diag;

@@ -17,6 +17,11 @@ constexpr int kThreads = 256;
 constexpr int kRBMin = 2;          // rows per wavefront of the row-blocked SpMM: 2, 4 or 8
 constexpr int kRowsMin = 32;       // smallest elementwise tile (rows per workgroup)
 constexpr int kRingCap = 1024;     // ring SpMM: x rows held per column in LDS (a power of 2)
+// Bytes of the newest basis vectors the batched update keeps cached: 7/8 of
+// MI355X's 256 MB last-level (memory-side) cache.  Those vectors are the
+// ones the next iterations re-read soonest; the older ones are streamed with
+// the nontemporal hint so that they do not evict them.
+constexpr size_t kUpdCacheBytes = (size_t)224 << 20;
 constexpr int kRingChunk = 128;    // ring SpMM: rows multiplied between two ring refills
 constexpr int kRingMaxRows = 4096; // ring SpMM: most rows per workgroup
 
@@ -34,6 +39,9 @@ struct Dev {
     int spmm_pipe;                  // batched SpMM: 1 = two gather buffers in flight
     int spmm_exact;                 // matrix-core SpMM: 1 = FP64 sums after every K-step
     int s0, sn;                     // batched kernels work on trajectory slots [s0, s0 + sn)
+    int upd_cached;                 // batched update: the newest upd_cached groups of 4 basis
+                                    // vectors are read with cached loads, older ones with
+                                    // nontemporal loads (kUpdCacheBytes)
     int S;                          // OU substeps
     int noise_model;                // DQ_NOISE_*: 0 OU, 1 HSR, 2 ZZ real, 3 none
     int zz_nosc;                    // ZZ oscillators per site

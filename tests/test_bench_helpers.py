@@ -20,7 +20,7 @@ def test_algorithmic_bytes():
 
 
 def test_kernel_names_and_traffic_lookup():
-    t = dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_pipeline=0, tile_rows=64)
+    t = dict(spmm_rows_per_wave=16, spmm_traj_per_lane=2, spmm_pipeline=0, tile_rows=128)
     assert bench.spmm_kernel_name(t) == "dq::k_spmm_mfma<4>"
     assert bench.spmm_kernel_name(dict(t, spmm_rows_per_wave=4)) == "dq::k_spmm_b<float,2,4,false>"
     up = bench.measured_traffic(bench.update_kernel_name(t))
@@ -29,5 +29,7 @@ def test_kernel_names_and_traffic_lookup():
     # traffic of a launch is at least its algorithmic bytes and within 30 % of them
     assert 1.0 <= up / (bench.update_bytes(12000, 256, 24, True) / 24) < 1.3
     assert 1.0 <= sp / bench.spmm_bytes(12000, 734160, 256, True) < 1.3
+    ring = bench.measured_traffic(bench.spmm_kernel_name(dict(t, spmm_window_rows=768)))
+    assert ring is not None and 1.0 <= ring / bench.spmm_bytes(12000, 734160, 256, True) < 1.5
     # no line for a choice that was not measured
     assert bench.measured_traffic(bench.update_kernel_name(dict(t, tile_rows=256))) is None
