@@ -590,6 +590,8 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
     d.sn = d.Bp;
     d.rt_s = plain ? 64 : 4 * d.rb;      // batched: one row block per wave
     d.ntile_e = (d.n + d.rt_e - 1) / d.rt_e;
+    d.rt_o = plain ? d.rt_e : dq::kRowsMin;
+    d.ntile_o = (d.n + d.rt_o - 1) / d.rt_o;
     d.ntile_s = (d.n + d.rt_s - 1) / d.rt_s;
     d.ntile_p = d.ntile_s;
     d.ring_w = s->ring_waves;

@@ -1121,7 +1121,9 @@ hipError_t launch_update_b(const Dev& d, int j, hipStream_t s)
 // Combine psi = y_0 psi + sum_{k>=1} (y_k s_k) v_k (propStep.cu:31-44) and the
 // partial |psi|^2; same 16 x 4 lane layout as the update.  TK: the Krylov part
 // of a Taylor-Krylov cycle, psi = t + sum_{k>=0} (y_k s_k) v_k
-// (taylorKrylovClass.cu:372-418).
+// (taylorKrylovClass.cu:372-418).  The basis is read with the nontemporal
+// hint: no kernel reads it again before the next cycle rewrites it, and the
+// hint keeps it from evicting psi, which the next prologue reads.
 // ---------------------------------------------------------------------------
 template <typename VT, int KW, bool TK>
 __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
@@ -1162,7 +1164,7 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
         for (int u = 0; u < KW; ++u) {
             const int k = kg + 4 * u;
-            if (k >= k0 && k < m) nk[u] = V[(size_t)k * d.vs + e];
+            if (k >= k0 && k < m) nk[u] = load_nt(&V[(size_t)k * d.vs + e]);
         }
         if (kg == 0) np = TK ? d.tk_t[e] : d.psi[e];
     }
@@ -1177,7 +1179,7 @@ __global__ __launch_bounds__(kThreads) void k_combine_b(Dev d)
 #pragma unroll
             for (int u = 0; u < KW; ++u) {
                 const int k = kg + 4 * u;
-                if (k >= k0 && k < m) nk[u] = V[(size_t)k * d.vs + en];
+                if (k >= k0 && k < m) nk[u] = load_nt(&V[(size_t)k * d.vs + en]);
             }
             if (kg == 0) np = TK ? d.tk_t[en] : d.psi[en];
         }

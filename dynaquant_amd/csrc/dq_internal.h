@@ -30,6 +30,11 @@ struct Dev {
     int mixed;                      // 1: float2 basis and FP32 SpMM
     int plain;                      // 1: B == 1 single-trajectory kernels
     int rt_e, ntile_e;              // elementwise kernels: rows per tile, tiles
+    int rt_o, ntile_o;              // prologue (noise, observables) and its reduction: rows
+                                    // per tile, tiles (kRowsMin-row tiles in the batched path:
+                                    // the noise recursion is latency-bound and needs the
+                                    // workgroups a concurrent slice of 64 trajectories lacks
+                                    // with longer tiles)
     int rt_s, ntile_s;              // SpMM kernels
     int ntile_p;                    // tiles of the SpMM partials in part_s (ntile_s, or the
                                     // ring kernel's row ranges when ring_rows > 0)

@@ -128,26 +128,27 @@ struct Tile {
     bool ok;
 };
 
+// tile of a workgroup over rows in tiles of rt (ntile of them) x 64 trajectories
 template <bool PLAIN>
-__device__ __forceinline__ Tile elem_tile(const Dev& d)
+__device__ __forceinline__ Tile elem_tile(const Dev& d, int rt, int ntile)
 {
     Tile t;
     const int ntt = PLAIN ? 1 : d.sn / 64;
-    const int nblk = d.ntile_e * ntt;
+    const int nblk = ntile * ntt;
     const int logical = xcd_tile(blockIdx.x, gridDim.x);
     t.ok = logical < nblk;
     t.rtile = logical / ntt;
     const int ttile = (PLAIN ? 0 : d.s0 / 64) + logical % ntt;
     const int tid = threadIdx.x;
     t.b = PLAIN ? 0 : ttile * 64 + (tid & 63);
-    t.r0 = t.rtile * d.rt_e;
-    t.r1 = min(d.n, t.r0 + d.rt_e);
+    t.r0 = t.rtile * rt;
+    t.r1 = min(d.n, t.r0 + rt);
     t.rstep = PLAIN ? kThreads : 4;
     t.rfirst = t.r0 + (PLAIN ? tid : (tid >> 6));
     return t;
 }
 
-static inline int elem_grid(const Dev& d) { return pad8(d.ntile_e * (d.plain ? 1 : d.sn / 64)); }
+static inline int elem_grid(const Dev& d, int ntile) { return pad8(ntile * (d.plain ? 1 : d.sn / 64)); }
 
 // ---------------------------------------------------------------------------
 // Static disorder and stationary OU start (systemClass.cu:759-804, 1028-1036)
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
 {
 #pragma clang fp contract(off)
     using V2 = typename Vec2<VT>::T;
-    Tile t = elem_tile<PLAIN>(d);
+    Tile t = elem_tile<PLAIN>(d, d.rt_o, d.ntile_o);
     if (!t.ok) return;
     const int b = t.b;
     const int64_t step = *d.step;
@@ -314,13 +315,13 @@ __global__ __launch_bounds__(kThreads) void k_prologue(Dev d, int renorm, int no
     if (grp || corr_on || pr_on) {
         // quantities: group g at q = g, C(n) at q = kMaxGroups, kMaxGroups + 1,
         // sum |c|^4 at q = kMaxGroups + 2
-        store_partials<PLAIN, kMaxGroups + 3>(acc, kMaxGroups + 3, d.part, d.ntile_e, t.rtile, d.Bp, b);
+        store_partials<PLAIN, kMaxGroups + 3>(acc, kMaxGroups + 3, d.part, d.ntile_o, t.rtile, d.Bp, b);
     }
 }
 
 hipError_t launch_prologue(const Dev& d, int renorm, int noise, int record, int copy_v0, hipStream_t s)
 {
-    dim3 grid(elem_grid(d)), block(kThreads);
+    dim3 grid(elem_grid(d, d.ntile_o)), block(kThreads);
     if (d.plain) {
         if (d.mixed) hipLaunchKernelGGL((k_prologue<true, float>), grid, block, 0, s, d, renorm, noise, record, copy_v0);
         else hipLaunchKernelGGL((k_prologue<true, double>), grid, block, 0, s, d, renorm, noise, record, copy_v0);
@@ -346,18 +347,18 @@ __global__ __launch_bounds__(kThreads) void k_reduce_obs(Dev d)
     const int b = PLAIN ? 0 : d.s0 + blockIdx.x * 64 + (threadIdx.x & 63);
     if (grp) {
         for (int g = 0; g < d.ngroups; ++g) {
-            double s = block_tile_sum<PLAIN>(d.part, g, d.ntile_e, d.Bp, b);
+            double s = block_tile_sum<PLAIN>(d.part, g, d.ntile_o, d.Bp, b);
             if (threadIdx.x < (PLAIN ? 1 : 64)) d.pops[((size_t)rec_idx * d.ngroups + g) * d.Bp + b] = s;
         }
     }
     if (corr_on) {
-        double re = block_tile_sum<PLAIN>(d.part, kMaxGroups, d.ntile_e, d.Bp, b);
-        double im = block_tile_sum<PLAIN>(d.part, kMaxGroups + 1, d.ntile_e, d.Bp, b);
+        double re = block_tile_sum<PLAIN>(d.part, kMaxGroups, d.ntile_o, d.Bp, b);
+        double im = block_tile_sum<PLAIN>(d.part, kMaxGroups + 1, d.ntile_o, d.Bp, b);
         if (threadIdx.x < (PLAIN ? 1 : 64)) d.corr[(size_t)step * d.Bp + b] = make_double2(re, im);
     }
     if (pr_on) {
         // the reference stores 1 / sum |c|^4 (systemClass.cu:2251-2252)
-        double s4 = block_tile_sum<PLAIN>(d.part, kMaxGroups + 2, d.ntile_e, d.Bp, b);
+        double s4 = block_tile_sum<PLAIN>(d.part, kMaxGroups + 2, d.ntile_o, d.Bp, b);
         if (threadIdx.x < (PLAIN ? 1 : 64)) d.partic[(size_t)step * d.Bp + b] = s4 > 0.0 ? 1.0 / s4 : 0.0;
     }
 }
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void k_update(Dev d, int j)
     using V2 = typename Vec2<VT>::T;
     __shared__ double2 s_c[MAXV][64];
     __shared__ double s_w[64];
-    Tile t = elem_tile<PLAIN>(d);
+    Tile t = elem_tile<PLAIN>(d, d.rt_e, d.ntile_e);
     if (!t.ok) return;
     const int b = t.b, bl = PLAIN ? 0 : (threadIdx.x & 63);
     const int Bp = d.Bp;
@@ -594,7 +595,7 @@ __global__ __launch_bounds__(kThreads) void k_update(Dev d, int j)
 template <bool PLAIN, typename VT>
 static void launch_update_t(const Dev& d, int j, hipStream_t s)
 {
-    dim3 grid(elem_grid(d)), block(kThreads);
+    dim3 grid(elem_grid(d, d.ntile_e)), block(kThreads);
     if (d.m <= 8) hipLaunchKernelGGL((k_update<PLAIN, VT, 8>), grid, block, 0, s, d, j);
     else if (d.m <= 16) hipLaunchKernelGGL((k_update<PLAIN, VT, 16>), grid, block, 0, s, d, j);
     else if (d.m <= 24) hipLaunchKernelGGL((k_update<PLAIN, VT, 24>), grid, block, 0, s, d, j);
@@ -992,7 +993,7 @@ __global__ __launch_bounds__(kThreads) void k_combine(Dev d)
 {
     using V2 = typename Vec2<VT>::T;
     __shared__ double2 s_y[MAXV][64];
-    Tile t = elem_tile<PLAIN>(d);
+    Tile t = elem_tile<PLAIN>(d, d.rt_e, d.ntile_e);
     if (!t.ok) return;
     const int b = t.b, bl = PLAIN ? 0 : (threadIdx.x & 63);
     const int Bp = d.Bp, m = d.m;
@@ -1032,7 +1033,7 @@ __global__ __launch_bounds__(kThreads) void k_combine(Dev d)
 template <bool PLAIN, typename VT>
 static void launch_combine_t(const Dev& d, hipStream_t s)
 {
-    dim3 grid(elem_grid(d)), block(kThreads);
+    dim3 grid(elem_grid(d, d.ntile_e)), block(kThreads);
     if (d.m <= 8) hipLaunchKernelGGL((k_combine<PLAIN, VT, 8>), grid, block, 0, s, d);
     else if (d.m <= 16) hipLaunchKernelGGL((k_combine<PLAIN, VT, 16>), grid, block, 0, s, d);
     else if (d.m <= 24) hipLaunchKernelGGL((k_combine<PLAIN, VT, 24>), grid, block, 0, s, d);
@@ -1183,7 +1184,7 @@ template <bool PLAIN, typename VT>
 __global__ __launch_bounds__(kThreads) void k_tk_scale(Dev d)
 {
     using V2 = typename Vec2<VT>::T;
-    Tile t = elem_tile<PLAIN>(d);
+    Tile t = elem_tile<PLAIN>(d, d.rt_e, d.ntile_e);
     if (!t.ok) return;
     const int b = t.b;
     const double nu = d.tk_nrm[b];
@@ -1203,7 +1204,7 @@ __global__ __launch_bounds__(kThreads) void k_tk_scale(Dev d)
 
 hipError_t launch_tk_start(const Dev& d, hipStream_t s)
 {
-    dim3 grid(elem_grid(d)), block(kThreads);
+    dim3 grid(elem_grid(d, d.ntile_e)), block(kThreads);
     if (d.plain) {
         hipLaunchKernelGGL((k_tk_norm<true>), dim3(1), block, 0, s, d);
         hipLaunchKernelGGL((k_tk_scale<true, double>), grid, block, 0, s, d);
@@ -1221,7 +1222,7 @@ __global__ __launch_bounds__(kThreads) void k_tk_combine(Dev d)
 {
     using V2 = typename Vec2<VT>::T;
     __shared__ double2 s_y[MAXV][64];
-    Tile t = elem_tile<PLAIN>(d);
+    Tile t = elem_tile<PLAIN>(d, d.rt_e, d.ntile_e);
     if (!t.ok) return;
     const int b = t.b, bl = PLAIN ? 0 : (threadIdx.x & 63);
     const int Bp = d.Bp, m = d.m;
@@ -1263,7 +1264,7 @@ __global__ __launch_bounds__(kThreads) void k_tk_combine(Dev d)
 template <bool PLAIN, typename VT>
 static void launch_tk_combine_t(const Dev& d, hipStream_t s)
 {
-    dim3 grid(elem_grid(d)), block(kThreads);
+    dim3 grid(elem_grid(d, d.ntile_e)), block(kThreads);
     if (d.m <= 8) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 8>), grid, block, 0, s, d);
     else if (d.m <= 16) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 16>), grid, block, 0, s, d);
     else if (d.m <= 24) hipLaunchKernelGGL((k_tk_combine<PLAIN, VT, 24>), grid, block, 0, s, d);
