@@ -241,6 +241,17 @@ struct Pending {
     }
 };
 
+// basis vector groups the batched update keeps cached (dq::kUpdCacheBytes):
+// a vector holds the active sites of every slot, so the budget is divided by
+// their sum (the rows a slot does not use are never read)
+static void set_upd_cached(dq_system* sys)
+{
+    size_t sites = 0;
+    for (int32_t na : sys->nact_slot) sites += (size_t)na;
+    const size_t group = 4 * std::max<size_t>(sites, 1) * (sys->d.mixed ? sizeof(float2) : sizeof(double2));
+    sys->d.upd_cached = (int)std::min<size_t>(dq::kMaxM / 4, dq::kUpdCacheBytes / group);
+}
+
 static void release_row_blocks(dq_system* sys, RowBlocks* b)
 {
     sys->release(b->ptr);
@@ -618,7 +629,6 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
 #define ALLOC(p, cnt) do { if ((st = s->alloc(p, cnt)) != DQ_OK) { delete s; return st; } } while (0)
     ALLOC(&psi, nB);
     d.vs = (nB + 63) / 64 * 64 + 32;
-    d.upd_cached = (int)std::min<size_t>(dq::kMaxM / 4, dq::kUpdCacheBytes / (4 * nB * (d.mixed ? 8 : 16)));
     if (d.mixed) { float2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; float* dg; ALLOC(&dg, nB); diag = dg; }
     else { double2* v; ALLOC(&v, d.vs * (s->m + 1)); V = v; double* dg; ALLOC(&dg, nB); diag = dg; }
     d.psi = psi; d.V = V; d.diag = diag;
@@ -654,6 +664,7 @@ dq_status dq_system_create(const dq_config* cfg, dq_system** out)
         s->slot.resize(s->B);
         for (int b = 0; b < s->B; ++b) s->slot[b] = b;
         s->nact_slot = hn;
+        set_upd_cached(s);
         hipMemcpyAsync(na, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
         hipMemcpyAsync(ti, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, s->stream);
         d.nact = na; d.traj_id = ti;
@@ -1009,6 +1020,7 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
         hn[k] = na[b];
     }
     sys->nact_slot = hn;
+    set_upd_cached(sys);
     sys->drop_graph();
     HIP_TRY(hipMemcpyAsync((void*)d.nact, hn.data(), sizeof(int32_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));
     HIP_TRY(hipMemcpyAsync((void*)d.traj_id, ht.data(), sizeof(uint64_t) * d.Bp, hipMemcpyHostToDevice, sys->stream));

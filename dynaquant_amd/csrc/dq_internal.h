@@ -46,7 +46,7 @@ struct Dev {
     int s0, sn;                     // batched kernels work on trajectory slots [s0, s0 + sn)
     int upd_cached;                 // batched update: the newest upd_cached groups of 4 basis
                                     // vectors are read with cached loads, older ones with
-                                    // nontemporal loads (kUpdCacheBytes)
+                                    // nontemporal loads (kUpdCacheBytes over the active sites)
     int S;                          // OU substeps
     int noise_model;                // DQ_NOISE_*: 0 OU, 1 HSR, 2 ZZ real, 3 none
     int zz_nosc;                    // ZZ oscillators per site
