@@ -135,6 +135,32 @@ def traffic_probe(dq, t, tuning, steps=2):
           file=sys.stderr, flush=True)
 
 
+def hbm_ceiling(reps=10):
+    """what plain streaming reaches on this GPU beyond the 256 MB last-level
+    cache, for context beside the 8 TB/s peak: a read-only reduction over a
+    4 GiB buffer and a 2 GiB -> 2 GiB copy (PyTorch's own kernels), best of
+    `reps` event-timed runs each"""
+    import torch
+    nbytes = 4 << 30
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty(nbytes // 8, dtype=torch.float32, device="cuda")
+    out = {}
+    for name, fn, moved in (("read_only_sum_4GiB", lambda: x.sum(), nbytes),
+                            ("copy_2GiB", lambda: y.copy_(x[: nbytes // 8]), nbytes)):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            best = min(best, a.elapsed_time(b))
+        out[name] = {"ms": best, "GB_per_s": moved / (best * 1e-3) / 1e9}
+    print(json.dumps({"hbm_ceiling": out, "peak_GB_per_s": 8000.0}), flush=True)
+
+
 def error_check(dq, t, tuning, steps, every=50, ntraj=64):
     """max |P_mixed - P_fp64| over every record of a `steps`-step run of the
     first `ntraj` trajectories: per site, and per group (2 walls x 4 axial
@@ -237,7 +263,12 @@ def main():
                     help="run only the short workload profiled for HBM traffic counters")
     ap.add_argument("--tuning", type=str, default=None,
                     help="launch choices as JSON (dq_tuning field names) instead of the autotuner")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="only time plain streaming (read-only sum, copy) beyond the last-level cache")
     args = ap.parse_args()
+    if args.hbm_ceiling:
+        hbm_ceiling()
+        return
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; trajectories shard across GPUs with no "
               "exchange, but only the single-GPU figure is measured here", file=sys.stderr)
