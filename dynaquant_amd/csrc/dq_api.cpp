@@ -1306,13 +1306,15 @@ extern "C" {
 static hipError_t enqueue_lanczos_pade(dq_system* sys, const Dev& d, hipStream_t st)
 {
     hipError_t e;
-    // batched: the SpMM partials of iteration j and the Gram partials of
-    // update j-1 are reduced in one launch before the scalars (the last
-    // update's |v_m|^2 is summed by the Pade kernel); plain: the scalars
+    // batched: the Gram partials of update j-1 are reduced by leading blocks of
+    // the matrix-core SpMM of iteration j, or by a launch of their own after
+    // the other SpMMs, and the scalars kernel sums the SpMM tiles itself (the
+    // last update's |v_m|^2 is summed by the Pade kernel); plain: the scalars
     // kernel sums the SpMM tiles and each update is followed by its reduction
     for (int j = 0; j < d.m; ++j) {
         if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_spmm(d, j, 0, st); })) != hipSuccess) return e;
-        if (!d.plain && (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce_all(d, j, st); })) != hipSuccess) return e;
+        if (!d.plain && j >= 1 && !dq::spmm_fuses_gram(d) &&
+            (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce_gram(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_SCALARS, [&] { return dq::launch_scalars_h(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_UPDATE, [&] { return dq::launch_update(d, j, st); })) != hipSuccess) return e;
         if (d.plain && (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
@@ -2322,9 +2324,8 @@ static dq_status apply_tuning(dq_system* sys, const dq_tuning& t)
 }
 
 // one Lanczos cycle without the Pade error sum, the renormalisation and the
-// observables: once the noise is set up, the noise prologue (whose workgroups
-// follow tile_rows like the update's), then SpMMs, scalars, updates,
-// reductions and the combine.  The caller saves and restores psi, the noise
+// observables: once the noise is set up, the noise prologue, then SpMMs,
+// reductions, scalars, updates and the combine.  The caller saves and restores psi, the noise
 // offsets and the diagonal around it.
 static bool tuning_with_noise(const dq_system* sys)
 {
@@ -2337,7 +2338,7 @@ static hipError_t enqueue_tuning_cycle(dq_system* sys, const Dev& d, hipStream_t
     if (tuning_with_noise(sys) && (e = dq::launch_prologue(d, 0, 1, 0, 1, st)) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
-        if (!d.plain && (e = dq::launch_reduce_all(d, j, st)) != hipSuccess) return e;
+        if (!d.plain && j >= 1 && !dq::spmm_fuses_gram(d) && (e = dq::launch_reduce_gram(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
         if (d.plain && (e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;

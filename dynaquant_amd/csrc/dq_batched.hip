@@ -12,6 +12,7 @@
 // one-lane-per-trajectory layout, so several waves per SIMD stay resident.
 #include "dq_device.h"
 #include "dq_internal.h"
+#include "dq_reduce.h"
 
 namespace dq {
 
@@ -392,8 +393,15 @@ __global__ __launch_bounds__(kThreads) void k_spmm_mfma(Dev d, int j, int last)
     constexpr int TL = NG / 2;                       // trajectories per lane
     constexpr int kDepth = 4;
     using XT = typename XV<NG>::T;
+    // leading blocks: the Gram-row reduction of update j-1
+    const int ntb = gram_task_blocks(d, j, last);
+    if ((int)blockIdx.x < ntb) {
+        __shared__ double s_red[16][64];
+        gram_task<4>(d, j, blockIdx.x, s_red);
+        return;
+    }
     const int ntt = d.sn / TW;
-    const int logical = xcd_tile(blockIdx.x, gridDim.x);
+    const int logical = xcd_tile(blockIdx.x - ntb, gridDim.x - ntb);
     if (logical >= d.ntile_s * ntt) return;
     // row-tile-major: the slices of a row block run on one XCD, so the block's
     // dense values (2.7x the nonzeros) come from HBM once per launch
@@ -610,7 +618,7 @@ hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s)
         // checks); spmm_traj_per_lane 1, 2, 4 = 16, 32, 64 trajectories per wave
         const int tl = d.spmm_tpl;
         const int tw = 16 * tl;
-        const dim3 grid(bpad8(d.ntile_s * (d.sn / tw))), block(kThreads);
+        const dim3 grid(gram_task_blocks(d, j, last) + bpad8(d.ntile_s * (d.sn / tw))), block(kThreads);
         if (d.spmm_exact) {
             if (tl == 4) hipLaunchKernelGGL((k_spmm_mfma<8, true>), grid, block, 0, s, d, j, last);
             else if (tl == 2) hipLaunchKernelGGL((k_spmm_mfma<4, true>), grid, block, 0, s, d, j, last);
@@ -733,21 +741,27 @@ hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/4) or 1).
-// Every block sums <v_j|w> over the SpMM tiles and rebuilds s_j and the Gram
-// row for its 64 trajectories; block y == 0 stores them; wave w of block y
-// computes the projection h_k, k = 4y + w < j.
+// Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/16) or
+// 1), 16 waves.  Every block sums <v_j|w> over the SpMM tiles itself (the
+// order of tile_sum16; tiles past the group's largest n_active hold zeros)
+// and rebuilds s_j and the Gram row for its 64 trajectories from the reduced
+// Gram sums of update j-1; block y == 0 stores them; wave w of block y
+// computes the projection h_k, k = 16y + w < j.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
+__global__ __launch_bounds__(1024) void k_scalars_hb(Dev d, int j)
 {
     __shared__ double s_sj[64];
     __shared__ double2 s_G[kMaxM + 1][64];
+    __shared__ double s_red[16][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = d.s0 + blockIdx.x * 64 + lane;
+    const int b0 = d.s0 + blockIdx.x * 64, b = b0 + lane;
     const int Bp = d.Bp;
-    // <v_j|w> reduced over the SpMM tiles by k_reduce_all into red[kRedC..]
-    const double cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
-    const double ci = d.red[((size_t)kRedC(d.m) + 1) * Bp + b];
+    const int maxn = min(d.n, uniform(d.nact[b0]));
+    const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
+    const int nts = min(d.ntile_p, (maxn + th - 1) / th);
+    const double cr = tile_sum16<16>(d.part_s + b, nts, Bp, s_red);
+    __syncthreads();                   // s_red is read by wave 0 above
+    const double ci = tile_sum16<16>(d.part_s + (size_t)d.ntile_p * Bp + b, nts, Bp, s_red);
     const bool writer = blockIdx.y == 0;
     if (wave == 0) {
         double sj = 1.0;
@@ -803,7 +817,7 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
         }
     }
     __syncthreads();
-    const int k = blockIdx.y * 4 + wave;
+    const int k = blockIdx.y * 16 + wave;
     if (k < j) {
         const double bk = (k == j - 1) ? s_sj[lane] : d.beta[(size_t)k * Bp + b];
         const double2 Gk1 = s_G[k + 1][lane];
@@ -833,23 +847,8 @@ __global__ __launch_bounds__(kThreads) void k_scalars_hb(Dev d, int j)
 __device__ __forceinline__ void wide_tile_sum(const double* p, int ntile, int Bp, double* out)
 {
     __shared__ double lds[16][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int t = wave;
-    for (; t + 112 < ntile; t += 128) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] += p[(size_t)(t + 16 * u) * Bp];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-        if (t + 16 * u < ntile) a[u] += p[(size_t)(t + 16 * u) * Bp];
-    lds[wave][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
-    __syncthreads();
-    if (wave == 0) {
-        double s = 0.0;
-        for (int w = 0; w < 16; ++w) s += lds[w][lane];
-        *out = s;
-    }
+    const double s = tile_sum16<16>(p, ntile, Bp, lds);
+    if ((threadIdx.x >> 6) == 0) *out = s;
 }
 
 // red[dst + q][b] = sum over tiles of part[q][tile][b]
@@ -861,30 +860,17 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
     wide_tile_sum(d.part + (size_t)q * ntile * Bp + b, ntile, Bp, d.red + ((size_t)dst + q) * Bp + b);
 }
 
-// blockIdx.y < 2: <v_j|w> (re, im) from the SpMM tiles; else quantity
-// y - 2 of update j-1 (Gram row re/im pairs, then |v_j|^2) from its tiles
-__global__ __launch_bounds__(1024) void k_reduce_all(Dev d, int j)
+// quantity blockIdx.y of update j-1 (Gram row re/im pairs, then |v_j|^2)
+// for one group of 64 slots, as the SpMM's leading blocks do it (gram_task)
+__global__ __launch_bounds__(1024) void k_reduce_gram(Dev d, int j)
 {
-    const int y = blockIdx.y;
-    const int b0 = d.s0 + blockIdx.x * 64, b = b0 + (threadIdx.x & 63);
-    const int Bp = d.Bp;
-    // tiles past the largest n_active of the 64 slots (the first, slots are
-    // sorted) hold exact zeros: summing only the others gives the same bits
-    const int maxn = min(d.n, (int)uniform(d.nact[b0]));
-    if (y < 2) {
-        const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
-        wide_tile_sum(d.part_s + (size_t)y * d.ntile_p * Bp + b, min(d.ntile_p, (maxn + th - 1) / th), Bp,
-                      d.red + ((size_t)kRedC(d.m) + y) * Bp + b);
-    } else {
-        wide_tile_sum(d.part + (size_t)(y - 2) * d.ntile_e * Bp + b, min(d.ntile_e, (maxn + d.rt_e - 1) / d.rt_e),
-                      Bp, d.red + (size_t)(y - 2) * Bp + b);
-    }
+    __shared__ double lds[16][64];
+    gram_task<16>(d, j, blockIdx.x * (2 * j + 1) + blockIdx.y, lds);
 }
 
-hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s)
+hipError_t launch_reduce_gram(const Dev& d, int j, hipStream_t s)
 {
-    const int nq = 2 + (j >= 1 ? 2 * j + 1 : 0);
-    hipLaunchKernelGGL(k_reduce_all, dim3(d.sn / 64, nq), dim3(1024), 0, s, d, j);
+    hipLaunchKernelGGL(k_reduce_gram, dim3(d.sn / 64, 2 * j + 1), dim3(1024), 0, s, d, j);
     return hipGetLastError();
 }
 
@@ -894,11 +880,12 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
     return hipGetLastError();
 }
 
-// after launch_reduce_all(d, j)
+// after the SpMM of iteration j (and launch_reduce_gram when it does not carry
+// the Gram reduction)
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 {
-    const int ny = j > 0 ? (j + 3) / 4 : 1;
-    hipLaunchKernelGGL(k_scalars_hb, dim3(d.sn / 64, ny), dim3(kThreads), 0, s, d, j);
+    const int ny = j > 0 ? (j + 15) / 16 : 1;
+    hipLaunchKernelGGL(k_scalars_hb, dim3(d.sn / 64, ny), dim3(1024), 0, s, d, j);
     return hipGetLastError();
 }
 
