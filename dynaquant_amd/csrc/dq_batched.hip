@@ -891,9 +891,9 @@ hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 
 // resident waves per SIMD asked of the register allocator for the FP32
 // instantiations (each fits without spilling; one more wave than the
-// allocator's own choice for KW = 3, 4, 6)
+// allocator's own choice for KW = 3, 4, 5, 6)
 template <typename VT, int KW>
-constexpr int kUpdateWaves = sizeof(VT) != 4 ? 1 : (KW <= 2 ? 8 : KW == 3 ? 6 : KW == 4 ? 5 : KW <= 6 ? 4 : 1);
+constexpr int kUpdateWaves = sizeof(VT) != 4 ? 1 : (KW <= 2 ? 8 : KW == 3 ? 6 : KW <= 5 ? 5 : KW == 6 ? 4 : 1);
 
 // a basis element read with the nontemporal hint
 template <typename T2>
