@@ -27,8 +27,10 @@ def test_kernel_names_and_traffic_lookup():
     sp = bench.measured_traffic(bench.spmm_kernel_name(t))
     assert up is not None and sp is not None
     # traffic of a launch is at least its algorithmic bytes and within 30 % of them
+    # (the matrix-core SpMM launch also reads the previous update's Gram tiles,
+    # 4.6 MB on average: within 40 %)
     assert 1.0 <= up / (bench.update_bytes(12000, 256, 24, True) / 24) < 1.3
-    assert 1.0 <= sp / bench.spmm_bytes(12000, 734160, 256, True) < 1.3
+    assert 1.0 <= sp / bench.spmm_bytes(12000, 734160, 256, True) < 1.4
     ring = bench.measured_traffic(bench.spmm_kernel_name(dict(t, spmm_window_rows=768)))
     assert ring is not None and 1.0 <= ring / bench.spmm_bytes(12000, 734160, 256, True) < 1.5
     # no line for a choice that was not measured
