@@ -639,6 +639,12 @@ hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s)
 // ---------------------------------------------------------------------------
 constexpr int kNMax = kMaxM + 2;
 constexpr int kNN = kNMax * kNMax;
+// threads of a Pade block: the Gauss-Jordan elimination of a pivot (n x 2n
+// complex elements) spread over 16 waves is a short chain per thread
+#ifndef PADE_THREADS
+#define PADE_THREADS 1024
+#endif
+constexpr int kPadeThreads = PADE_THREADS;
 
 // augmented tridiagonal of size m + aug (gpuLanczosHelper.cu:14-60): T, a 1
 // at (0, m) and ones on the superdiagonal of the trailing aug x aug block;
@@ -722,6 +728,7 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     double* X1r = sm + 2 * kNN;
     __shared__ int s_scale;
     __shared__ int s_piv;
+    __shared__ double s_inv[2];
     __shared__ double2 s_f[kNMax];
     const double cim = -theta;            // A = i cim Tbar  (propagate.cu:91-92)
     // infinity norm: row sums of |A_ij| in column order, as the oracle does,
@@ -773,11 +780,11 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     // Gauss-Jordan with partial pivoting (|re|+|im|, first maximum).  Each
     // thread owns the same elements of [X0 | Bm] at every pivot: their (row,
     // column) pairs are computed once, outside the pivot loop.
-    constexpr int kPer = (2 * kNMax * kNMax + kThreads - 1) / kThreads;
+    constexpr int kPer = (2 * kNMax * kNMax + kPadeThreads - 1) / kPadeThreads;
     int er[kPer], ec[kPer];
 #pragma unroll
     for (int r = 0; r < kPer; ++r) {
-        const int e = threadIdx.x + r * kThreads;
+        const int e = threadIdx.x + r * kPadeThreads;
         er[r] = e < n * 2 * n ? e / (2 * n) : -1;
         ec[r] = e < n * 2 * n ? e % (2 * n) : 0;
     }
@@ -807,10 +814,14 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
             }
             __syncthreads();
         }
-        const double2 pv = X0[k * n + k];
-        const double den = pv.x * pv.x + pv.y * pv.y;
-        const double ir = pv.x / den, ii = -pv.y / den;
+        if (threadIdx.x == 0) {
+            const double2 pv = X0[k * n + k];
+            const double den = pv.x * pv.x + pv.y * pv.y;
+            s_inv[0] = pv.x / den;
+            s_inv[1] = -pv.y / den;
+        }
         __syncthreads();
+        const double ir = s_inv[0], ii = s_inv[1];
         for (int jj = threadIdx.x; jj < 2 * n; jj += blockDim.x) {
             double2* row = jj < n ? &X0[k * n + jj] : &Bm[k * n + (jj - n)];
             const double2 a = *row;
@@ -858,7 +869,7 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     return s;
 }
 
-__global__ __launch_bounds__(kThreads) void k_pade(Dev d)
+__global__ __launch_bounds__(kPadeThreads) void k_pade(Dev d)
 {
     extern __shared__ double sm[];
     __shared__ double al[kMaxM], be[kMaxM];
@@ -869,16 +880,19 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
     // the last update's tiles (the plain path has it reduced in red[2m])
     double hq2 = 0.0, nu_b = 0.0;
     {
+        // the first four waves, in the same order whatever the block size
         const double* ps = d.plain ? d.part : d.part_s;
         double s = 0.0, v = 0.0;
-        for (int t = threadIdx.x; t < d.ntile_p; t += kThreads) s += ps[(size_t)t * Bp + b];
-        if (!d.plain)
-            for (int t = threadIdx.x; t < d.ntile_e; t += kThreads)
-                v += d.part[((size_t)(2 * m) * d.ntile_e + t) * Bp + b];
+        if (threadIdx.x < kThreads) {
+            for (int t = threadIdx.x; t < d.ntile_p; t += kThreads) s += ps[(size_t)t * Bp + b];
+            if (!d.plain)
+                for (int t = threadIdx.x; t < d.ntile_e; t += kThreads)
+                    v += d.part[((size_t)(2 * m) * d.ntile_e + t) * Bp + b];
+        }
         s = wave_sum(s);
         v = wave_sum(v);
         __shared__ double l4[2][4];
-        if ((threadIdx.x & 63) == 0) { l4[0][threadIdx.x >> 6] = s; l4[1][threadIdx.x >> 6] = v; }
+        if ((threadIdx.x & 63) == 0 && threadIdx.x < kThreads) { l4[0][threadIdx.x >> 6] = s; l4[1][threadIdx.x >> 6] = v; }
         __syncthreads();
         hq2 = (l4[0][0] + l4[0][1]) + (l4[0][2] + l4[0][3]);
         nu_b = (l4[1][0] + l4[1][1]) + (l4[1][2] + l4[1][3]);
@@ -899,20 +913,20 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
         s_hq = sm_ * __builtin_sqrt(hq2);
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < m; k += kThreads) {
+    for (int k = threadIdx.x; k < m; k += kPadeThreads) {
         al[k] = d.alpha[(size_t)k * Bp + b];
         be[k] = d.beta[(size_t)k * Bp + b];
     }
     __syncthreads();
     if (b >= d.B) {
-        for (int k = threadIdx.x; k < m; k += kThreads) d.ycoef[(size_t)k * Bp + b] = make_double2(0.0, 0.0);
+        for (int k = threadIdx.x; k < m; k += kPadeThreads) d.ycoef[(size_t)k * Bp + b] = make_double2(0.0, 0.0);
         return;
     }
     const double2* E;
     const int s = pade_block(m, d.aug, al, be, d.theta, sm, &E);
     if (s < 0) {
         if (threadIdx.x == 0) atomicOr(d.status, (int)kStatusScaling);
-        for (int k = threadIdx.x; k < m; k += kThreads) d.ycoef[(size_t)k * Bp + b] = make_double2(0.0, 0.0);
+        for (int k = threadIdx.x; k < m; k += kPadeThreads) d.ycoef[(size_t)k * Bp + b] = make_double2(0.0, 0.0);
         return;
     }
     const int n = m + d.aug;
@@ -920,7 +934,7 @@ __global__ __launch_bounds__(kThreads) void k_pade(Dev d)
     // the rows 0..m-1 of column m + p - 1 (taylorKrylovClass.cu:372-397)
     const int ecol = d.tk ? n - 1 : 0;
     const double f = d.tk ? d.tk_nrm[b] : 1.0;
-    for (int k = threadIdx.x; k < m; k += kThreads) {
+    for (int k = threadIdx.x; k < m; k += kPadeThreads) {
         const double2 y = E[k * n + ecol];
         const double sk = d.sc[(size_t)k * Bp + b] * f;
         d.ycoef[(size_t)k * Bp + b] = make_double2(y.x * sk, y.y * sk);
@@ -948,17 +962,17 @@ static size_t pade_lds_bytes() { return sizeof(double) * 6 * kNN; }
 
 hipError_t launch_pade(const Dev& d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pade, dim3(d.sn), dim3(kThreads), pade_lds_bytes(), s, d);
+    hipLaunchKernelGGL(k_pade, dim3(d.sn), dim3(kPadeThreads), pade_lds_bytes(), s, d);
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kThreads) void k_pade_tbar(int m, const double* alpha, const double* beta,
+__global__ __launch_bounds__(kPadeThreads) void k_pade_tbar(int m, const double* alpha, const double* beta,
                                                          double theta, double2* Eout, int* s_out)
 {
     extern __shared__ double sm[];
     __shared__ double al[kMaxM], be[kMaxM];
     const int c = blockIdx.x;
-    for (int k = threadIdx.x; k < m; k += kThreads) {
+    for (int k = threadIdx.x; k < m; k += kPadeThreads) {
         al[k] = alpha[(size_t)c * m + k];
         be[k] = beta[(size_t)c * m + k];
     }
@@ -969,7 +983,7 @@ __global__ __launch_bounds__(kThreads) void k_pade_tbar(int m, const double* alp
     const int n = m + 2;
     if (s < 0) return;
     // column-major output
-    for (int e = threadIdx.x; e < n * n; e += kThreads) {
+    for (int e = threadIdx.x; e < n * n; e += kPadeThreads) {
         const int i = e / n, jj = e % n;
         Eout[(size_t)c * n * n + (size_t)jj * n + i] = E[e];
     }
@@ -978,7 +992,7 @@ __global__ __launch_bounds__(kThreads) void k_pade_tbar(int m, const double* alp
 hipError_t launch_pade_tbar(int m, int count, const double* alpha, const double* beta,
                             double theta, double2* E, int* s_out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pade_tbar, dim3(count), dim3(kThreads), pade_lds_bytes(), s, m, alpha, beta,
+    hipLaunchKernelGGL(k_pade_tbar, dim3(count), dim3(kPadeThreads), pade_lds_bytes(), s, m, alpha, beta,
                        theta, E, s_out);
     return hipGetLastError();
 }
