@@ -1307,14 +1307,15 @@ static hipError_t enqueue_lanczos_pade(dq_system* sys, const Dev& d, hipStream_t
 {
     hipError_t e;
     // batched: the Gram partials of update j-1 are reduced by leading blocks of
-    // the matrix-core SpMM of iteration j, or by a launch of their own after
-    // the other SpMMs, and the scalars kernel sums the SpMM tiles itself (the
-    // last update's |v_m|^2 is summed by the Pade kernel); plain: the scalars
-    // kernel sums the SpMM tiles and each update is followed by its reduction
+    // the matrix-core SpMM of iteration j and the scalars kernel sums the SpMM
+    // tiles itself, or both are reduced by one launch after the other SpMMs
+    // (the last update's |v_m|^2 is summed by the Pade kernel); plain: the
+    // scalars kernel sums the SpMM tiles and each update is followed by its
+    // reduction
     for (int j = 0; j < d.m; ++j) {
         if ((e = timed(sys, DQ_K_SPMM, [&] { return dq::launch_spmm(d, j, 0, st); })) != hipSuccess) return e;
-        if (!d.plain && j >= 1 && !dq::spmm_fuses_gram(d) &&
-            (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce_gram(d, j, st); })) != hipSuccess) return e;
+        if (!d.plain && !dq::spmm_fuses_gram(d) &&
+            (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce_all(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_SCALARS, [&] { return dq::launch_scalars_h(d, j, st); })) != hipSuccess) return e;
         if ((e = timed(sys, DQ_K_UPDATE, [&] { return dq::launch_update(d, j, st); })) != hipSuccess) return e;
         if (d.plain && (e = timed(sys, DQ_K_REDUCE, [&] { return dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st); })) != hipSuccess) return e;
@@ -2338,7 +2339,7 @@ static hipError_t enqueue_tuning_cycle(dq_system* sys, const Dev& d, hipStream_t
     if (tuning_with_noise(sys) && (e = dq::launch_prologue(d, 0, 1, 0, 1, st)) != hipSuccess) return e;
     for (int j = 0; j < d.m; ++j) {
         if ((e = dq::launch_spmm(d, j, 0, st)) != hipSuccess) return e;
-        if (!d.plain && j >= 1 && !dq::spmm_fuses_gram(d) && (e = dq::launch_reduce_gram(d, j, st)) != hipSuccess) return e;
+        if (!d.plain && !dq::spmm_fuses_gram(d) && (e = dq::launch_reduce_all(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_scalars_h(d, j, st)) != hipSuccess) return e;
         if ((e = dq::launch_update(d, j, st)) != hipSuccess) return e;
         if (d.plain && (e = dq::launch_reduce(d, 2 * (j + 1) + 1, d.ntile_e, st)) != hipSuccess) return e;

@@ -741,27 +741,36 @@ hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
-// Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/16) or
-// 1), 16 waves.  Every block sums <v_j|w> over the SpMM tiles itself (the
-// order of tile_sum16; tiles past the group's largest n_active hold zeros)
-// and rebuilds s_j and the Gram row for its 64 trajectories from the reduced
-// Gram sums of update j-1; block y == 0 stores them; wave w of block y
-// computes the projection h_k, k = 16y + w < j.
+// Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/NW) or
+// 1), NW waves.  Every block takes <v_j|w> -- FUSED (after the matrix-core
+// SpMM, NW = 16): summed over the SpMM tiles by the block itself; else from
+// red[kRedC..] (k_reduce_all, NW = 4), the same order of tile_sum16 either
+// way -- and rebuilds s_j and the Gram row for its 64 trajectories from the
+// reduced Gram sums of update j-1; block y == 0 stores them; wave w of block
+// y computes the projection h_k, k = NW y + w < j.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scalars_hb(Dev d, int j)
+template <bool FUSED>
+__global__ __launch_bounds__(FUSED ? 1024 : 256) void k_scalars_hb(Dev d, int j)
 {
+    constexpr int NW = FUSED ? 16 : 4;
     __shared__ double s_sj[64];
     __shared__ double2 s_G[kMaxM + 1][64];
-    __shared__ double s_red[16][64];
+    __shared__ double s_red[FUSED ? 16 : 1][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b0 = d.s0 + blockIdx.x * 64, b = b0 + lane;
     const int Bp = d.Bp;
-    const int maxn = min(d.n, uniform(d.nact[b0]));
-    const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
-    const int nts = min(d.ntile_p, (maxn + th - 1) / th);
-    const double cr = tile_sum16<16>(d.part_s + b, nts, Bp, s_red);
-    __syncthreads();                   // s_red is read by wave 0 above
-    const double ci = tile_sum16<16>(d.part_s + (size_t)d.ntile_p * Bp + b, nts, Bp, s_red);
+    double cr, ci;
+    if constexpr (FUSED) {
+        const int maxn = min(d.n, uniform(d.nact[b0]));
+        const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
+        const int nts = min(d.ntile_p, (maxn + th - 1) / th);
+        cr = tile_sum16<16>(d.part_s + b, nts, Bp, s_red);
+        __syncthreads();                   // s_red is read by wave 0 above
+        ci = tile_sum16<16>(d.part_s + (size_t)d.ntile_p * Bp + b, nts, Bp, s_red);
+    } else {
+        cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
+        ci = d.red[((size_t)kRedC(d.m) + 1) * Bp + b];
+    }
     const bool writer = blockIdx.y == 0;
     if (wave == 0) {
         double sj = 1.0;
@@ -817,7 +826,7 @@ __global__ __launch_bounds__(1024) void k_scalars_hb(Dev d, int j)
         }
     }
     __syncthreads();
-    const int k = blockIdx.y * 16 + wave;
+    const int k = blockIdx.y * NW + wave;
     if (k < j) {
         const double bk = (k == j - 1) ? s_sj[lane] : d.beta[(size_t)k * Bp + b];
         const double2 Gk1 = s_G[k + 1][lane];
@@ -860,17 +869,30 @@ __global__ __launch_bounds__(1024) void k_reduce_wide(Dev d, int ntile, int dst)
     wide_tile_sum(d.part + (size_t)q * ntile * Bp + b, ntile, Bp, d.red + ((size_t)dst + q) * Bp + b);
 }
 
-// quantity blockIdx.y of update j-1 (Gram row re/im pairs, then |v_j|^2)
-// for one group of 64 slots, as the SpMM's leading blocks do it (gram_task)
-__global__ __launch_bounds__(1024) void k_reduce_gram(Dev d, int j)
+// after the SpMMs that do not carry the Gram reduction: blockIdx.y < 2:
+// <v_j|w> (re, im) from the SpMM tiles into red[kRedC..]; else quantity y - 2
+// of update j-1 (Gram row re/im pairs, then |v_j|^2) for one group of 64
+// slots, as the matrix-core SpMM's leading blocks do it (gram_task)
+__global__ __launch_bounds__(1024) void k_reduce_all(Dev d, int j)
 {
     __shared__ double lds[16][64];
-    gram_task<16>(d, j, blockIdx.x * (2 * j + 1) + blockIdx.y, lds);
+    const int y = blockIdx.y;
+    if (y >= 2) {
+        gram_task<16>(d, j, blockIdx.x * (2 * j + 1) + (y - 2), lds);
+        return;
+    }
+    const int b0 = d.s0 + blockIdx.x * 64, b = b0 + (threadIdx.x & 63);
+    const int maxn = min(d.n, uniform(d.nact[b0]));
+    const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
+    const double v = tile_sum16<16>(d.part_s + (size_t)y * d.ntile_p * d.Bp + b, min(d.ntile_p, (maxn + th - 1) / th),
+                                    d.Bp, lds);
+    if ((threadIdx.x >> 6) == 0) d.red[((size_t)kRedC(d.m) + y) * d.Bp + b] = v;
 }
 
-hipError_t launch_reduce_gram(const Dev& d, int j, hipStream_t s)
+hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_gram, dim3(d.sn / 64, 2 * j + 1), dim3(1024), 0, s, d, j);
+    const int nq = 2 + (j >= 1 ? 2 * j + 1 : 0);
+    hipLaunchKernelGGL(k_reduce_all, dim3(d.sn / 64, nq), dim3(1024), 0, s, d, j);
     return hipGetLastError();
 }
 
@@ -880,12 +902,17 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
     return hipGetLastError();
 }
 
-// after the SpMM of iteration j (and launch_reduce_gram when it does not carry
+// after the SpMM of iteration j (and launch_reduce_all when it does not carry
 // the Gram reduction)
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 {
-    const int ny = j > 0 ? (j + 15) / 16 : 1;
-    hipLaunchKernelGGL(k_scalars_hb, dim3(d.sn / 64, ny), dim3(1024), 0, s, d, j);
+    if (spmm_fuses_gram(d)) {
+        const int ny = j > 0 ? (j + 15) / 16 : 1;
+        hipLaunchKernelGGL(k_scalars_hb<true>, dim3(d.sn / 64, ny), dim3(1024), 0, s, d, j);
+    } else {
+        const int ny = j > 0 ? (j + 3) / 4 : 1;
+        hipLaunchKernelGGL(k_scalars_hb<false>, dim3(d.sn / 64, ny), dim3(kThreads), 0, s, d, j);
+    }
     return hipGetLastError();
 }
 

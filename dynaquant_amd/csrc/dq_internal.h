@@ -143,10 +143,15 @@ __host__ __device__ inline bool obs_on(const Dev& d)
 
 enum StatusBits { kStatusScaling = 1, kStatusNonfinite = 2 };
 
+// first row of red[] holding the reduced <v_j|w> when a reduction launch sums
+// it (after the Gram sums)
+__host__ __device__ inline int kRedC(int m) { return 2 * (m + 1) + 1; }
+
 // Batched Lanczos iteration j: the matrix-core SpMM carries the reduction of
 // update j-1's Gram row (2j+1 quantities per group of 64 slots) in leading
 // blocks of its own launch, padded to a multiple of 8 so that its row blocks
-// keep their XCDs; the other SpMMs leave it to launch_reduce_gram.
+// keep their XCDs (and the scalars kernel sums the SpMM tiles); the other
+// SpMMs leave both to launch_reduce_all.
 __host__ __device__ inline bool spmm_fuses_gram(const Dev& d) { return !d.plain && d.rb == 16 && d.ring_rows <= 0; }
 __host__ __device__ inline int gram_task_blocks(const Dev& d, int j, int last)
 {
@@ -165,10 +170,11 @@ hipError_t launch_reduce(const Dev& d, int nq, int ntile, hipStream_t s);
 hipError_t launch_spmm_b(const Dev& d, int j, int last, hipStream_t s);
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s);
 hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStream_t s);
-// Lanczos iteration j >= 1, batched, when the SpMM does not carry it
-// (spmm_fuses_gram): the Gram row and |v_j|^2 of update j-1 (part, 2j+1
-// quantities) into red[0..2j]
-hipError_t launch_reduce_gram(const Dev& d, int j, hipStream_t s);
+// Lanczos iteration j, batched, when the SpMM does not carry the Gram
+// reduction (spmm_fuses_gram): <v_j|w> over the SpMM tiles (part_s) into
+// red[kRedC(m)..+1] and, for j >= 1, the Gram row and |v_j|^2 of update j-1
+// (part, 2j+1 quantities) into red[0..2j], in one launch
+hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
 // ring SpMM (dq_ring.hip)
 hipError_t launch_spmm_ring(const Dev& d, int j, int last, hipStream_t s);
