@@ -241,6 +241,34 @@ struct Pending {
     }
 };
 
+// Slots of a sorted batch regrouped so that concurrent trajectory slices
+// (contiguous runs of whole 64-slot groups) carry similar site counts: the
+// 64-slot groups g0 >= g1 >= ... are dealt over four bins in snake order
+// (g0 g1 g2 g3 g3' g2' ...: bin 0 takes g0, g7, g8, g15, ...), and the bins are
+// laid out as 0, 3, 1, 2, so that two slices hold bins {0, 3} and {1, 2} and
+// four slices one bin each.  Every aligned run of 64 or 128 slots still starts
+// with its largest n_active (a bin's groups stay in descending order, and with
+// one group per bin the pairs are (g0, g3) and (g1, g2)), which the kernels use
+// to bound their rows.  Only for a batch without padding slots (B == Bp) of 4
+// or a multiple of 8 groups; otherwise the sorted order stays.  Results per
+// trajectory do not depend on its slot.
+static std::vector<int> balanced_group_order(const std::vector<int>& sorted, int B, int Bp)
+{
+    const int G = B / 64;
+    if (B != Bp || B % 64 != 0 || !(G == 4 || (G >= 8 && G % 8 == 0))) return sorted;
+    std::vector<std::vector<int>> bin(4);
+    for (int g = 0; g < G; ++g) {
+        const int r = g % 8;
+        bin[r < 4 ? r : 7 - r].push_back(g);
+    }
+    std::vector<int> out;
+    out.reserve(B);
+    for (int k : {0, 3, 1, 2})
+        for (int g : bin[k])
+            for (int q = 0; q < 64; ++q) out.push_back(sorted[g * 64 + q]);
+    return out;
+}
+
 // basis vector groups the batched update keeps cached (dq::kUpdCacheBytes):
 // a vector holds the active sites of every slot, so the budget is divided by
 // their sum (the rows a slot does not use are never read)
@@ -1011,6 +1039,7 @@ dq_status dq_system_set_trajectories(dq_system* sys, const uint64_t* traj_ids, c
     std::vector<int> order(B);
     for (int b = 0; b < B; ++b) order[b] = b;
     if (!d.plain) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return na[x] > na[y]; });
+    if (!d.plain) order = balanced_group_order(order, B, d.Bp);
     std::vector<int32_t> hn(d.Bp, 0);
     std::vector<uint64_t> ht(d.Bp, 0);
     for (int k = 0; k < B; ++k) {
