@@ -742,93 +742,95 @@ hipError_t launch_tk_power_b(const Dev& d, int k, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // Per-trajectory scalars of iteration j (batched): grid (Bp/64, ceil(j/NW) or
-// 1), NW waves.  Every block takes <v_j|w> -- FUSED (after the matrix-core
-// SpMM, NW = 16): summed over the SpMM tiles by the block itself; else from
-// red[kRedC..] (k_reduce_all, NW = 4), the same order of tile_sum16 either
-// way -- and rebuilds s_j and the Gram row for its 64 trajectories from the
-// reduced Gram sums of update j-1; block y == 0 stores them; wave w of block
-// y computes the projection h_k, k = NW y + w < j.
+// 1), NW waves.  Block y == 0 takes <v_j|w> -- FUSED (after the matrix-core
+// SpMM, NW = 8): summed over the SpMM tiles by the block itself, both halves
+// at once; else from red[kRedC..] (k_reduce_all, NW = 4), the same order of
+// tile_sum16 either way -- and stores alpha_j, beta_{j-1}, s_j and the Gram
+// row of its 64 trajectories, rebuilt from the reduced Gram sums of update
+// j-1.  Every block rebuilds s_j and the Gram row for itself (wave w the
+// entries l in [8w, 8w + 8)); wave w of block y then computes the projection
+// h_k, k = NW y + w < j, which needs no <v_j|w>.  The 512-thread fused block
+// (not 1024) finds room on CUs busy with the other slices' updates sooner.
 // ---------------------------------------------------------------------------
 template <bool FUSED>
-__global__ __launch_bounds__(FUSED ? 1024 : 256) void k_scalars_hb(Dev d, int j)
+__global__ __launch_bounds__(FUSED ? 512 : 256) void k_scalars_hb(Dev d, int j)
 {
-    constexpr int NW = FUSED ? 16 : 4;
-    __shared__ double s_sj[64];
+    constexpr int NW = FUSED ? 8 : 4;
     __shared__ double2 s_G[kMaxM + 1][64];
-    __shared__ double s_red[FUSED ? 16 : 1][64];
+    __shared__ double s_red[FUSED ? 32 : 1][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b0 = d.s0 + blockIdx.x * 64, b = b0 + lane;
     const int Bp = d.Bp;
-    double cr, ci;
+    double cr = 0.0, ci = 0.0;
     if constexpr (FUSED) {
-        const int maxn = min(d.n, uniform(d.nact[b0]));
-        const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
-        const int nts = min(d.ntile_p, (maxn + th - 1) / th);
-        cr = tile_sum16<16>(d.part_s + b, nts, Bp, s_red);
-        __syncthreads();                   // s_red is read by wave 0 above
-        ci = tile_sum16<16>(d.part_s + (size_t)d.ntile_p * Bp + b, nts, Bp, s_red);
+        if (blockIdx.y == 0) {
+            const int maxn = min(d.n, uniform(d.nact[b0]));
+            const int th = d.ring_rows > 0 ? d.ring_rows : d.rt_s;
+            const int nts = min(d.ntile_p, (maxn + th - 1) / th);
+            tile_sum16_pair<NW>(d.part_s + b, d.part_s + (size_t)d.ntile_p * Bp + b, nts, Bp, s_red, cr, ci);
+        }
     } else {
         cr = d.red[((size_t)kRedC(d.m)) * Bp + b];
         ci = d.red[((size_t)kRedC(d.m) + 1) * Bp + b];
     }
-    const bool writer = blockIdx.y == 0;
-    if (wave == 0) {
-        double sj = 1.0;
-        if (j == 0) {
-            s_G[0][lane] = make_double2(1.0, 0.0);
-            if (writer) { d.sc[b] = 1.0; d.broke[b] = 0; }
+    const bool store = blockIdx.y == 0 && wave == 0;
+    // s_j = 1 / beta_{j-1}: every wave from the same loads (broke only ever
+    // goes 0 -> 1 on this very test, so a wave that reads the flag after wave
+    // 0 stored it reaches the same result)
+    double sj = 1.0, bet = 0.0;
+    if (j == 0) {
+        if (store) { d.sc[b] = 1.0; d.broke[b] = 0; }
+    } else {
+        const double nu = d.red[(size_t)(2 * j) * Bp + b];
+        bet = __builtin_sqrt(nu);
+        const double bprev = j >= 2 ? d.beta[(size_t)(j - 2) * Bp + b] : 0.0;
+        const double aprev = d.alpha[(size_t)(j - 1) * Bp + b];
+        int broke = d.broke[b];
+        if (broke || !(bet > d.tol * (__builtin_fabs(aprev) + bprev))) {
+            broke = 1;
+            bet = 0.0;
+            sj = 0.0;
         } else {
-            const double nu = d.red[(size_t)(2 * j) * Bp + b];
-            double bet = __builtin_sqrt(nu);
-            const double bprev = j >= 2 ? d.beta[(size_t)(j - 2) * Bp + b] : 0.0;
-            const double aprev = d.alpha[(size_t)(j - 1) * Bp + b];
-            int broke = d.broke[b];
-            if (broke || !(bet > d.tol * (__builtin_fabs(aprev) + bprev))) {
-                broke = 1;
-                bet = 0.0;
-                sj = 0.0;
-            } else {
-                sj = 1.0 / bet;
-            }
-            if (writer) {
-                d.broke[b] = broke;
-                d.beta[(size_t)(j - 1) * Bp + b] = bet;
-                d.sc[(size_t)j * Bp + b] = sj;
-            }
-            // eight l per round: the loads of a round are issued together
-            for (int l0 = 0; l0 < j; l0 += 8) {
-                double sl[8], gr[8], gi[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int l = min(l0 + u, j - 1);
-                    sl[u] = d.sc[(size_t)l * Bp + b];
-                    gr[u] = d.red[(size_t)(2 * l) * Bp + b];
-                    gi[u] = d.red[(size_t)(2 * l + 1) * Bp + b];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int l = l0 + u;
-                    if (l < j) {
-                        const double f = sl[u] * sj;
-                        const double2 G = make_double2(f * gr[u], f * gi[u]);
-                        s_G[l][lane] = G;
-                        if (writer) d.gram[(size_t)l * Bp + b] = G;
-                    }
-                }
-            }
-            s_G[j][lane] = make_double2(1.0, 0.0);
-            s_sj[lane] = bet;          // beta_{j-1}, needed by k = j-1 below
+            sj = 1.0 / bet;
         }
-        if (writer) {
-            const double hr = sj * sj * cr, hi = sj * sj * ci;
-            d.alpha[(size_t)j * Bp + b] = hr;
-            d.hco[((size_t)j * kMaxM + j) * Bp + b] = make_double2(hr, hi);
+        if (store) {
+            d.broke[b] = broke;
+            d.beta[(size_t)(j - 1) * Bp + b] = bet;
+            d.sc[(size_t)j * Bp + b] = sj;
         }
     }
+    if (store) {
+        const double hr = sj * sj * cr, hi = sj * sj * ci;
+        d.alpha[(size_t)j * Bp + b] = hr;
+        d.hco[((size_t)j * kMaxM + j) * Bp + b] = make_double2(hr, hi);
+    }
+    // Gram row of v_j rescaled: wave w takes l in [8 w, 8 w + 8) (+ 8 NW r),
+    // the eight loads of a group issued together
+    for (int l0 = 8 * wave; l0 < j; l0 += 8 * NW) {
+        double sl[8], gr[8], gi[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int l = min(l0 + u, j - 1);
+            sl[u] = d.sc[(size_t)l * Bp + b];
+            gr[u] = d.red[(size_t)(2 * l) * Bp + b];
+            gi[u] = d.red[(size_t)(2 * l + 1) * Bp + b];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int l = l0 + u;
+            if (l < j) {
+                const double f = sl[u] * sj;
+                const double2 G = make_double2(f * gr[u], f * gi[u]);
+                s_G[l][lane] = G;
+                if (blockIdx.y == 0) d.gram[(size_t)l * Bp + b] = G;
+            }
+        }
+    }
+    if (wave == 0) s_G[j][lane] = make_double2(1.0, 0.0);
     __syncthreads();
     const int k = blockIdx.y * NW + wave;
     if (k < j) {
-        const double bk = (k == j - 1) ? s_sj[lane] : d.beta[(size_t)k * Bp + b];
+        const double bk = (k == j - 1) ? bet : d.beta[(size_t)k * Bp + b];
         const double2 Gk1 = s_G[k + 1][lane];
         double sr = bk * Gk1.x, si = bk * Gk1.y;
         // h^(k)_l in rounds of eight independent loads, summed in order of l
@@ -907,8 +909,8 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
 hipError_t launch_scalars_hb(const Dev& d, int j, hipStream_t s)
 {
     if (spmm_fuses_gram(d)) {
-        const int ny = j > 0 ? (j + 15) / 16 : 1;
-        hipLaunchKernelGGL(k_scalars_hb<true>, dim3(d.sn / 64, ny), dim3(1024), 0, s, d, j);
+        const int ny = j > 0 ? (j + 7) / 8 : 1;
+        hipLaunchKernelGGL(k_scalars_hb<true>, dim3(d.sn / 64, ny), dim3(512), 0, s, d, j);
     } else {
         const int ny = j > 0 ? (j + 3) / 4 : 1;
         hipLaunchKernelGGL(k_scalars_hb<false>, dim3(d.sn / 64, ny), dim3(kThreads), 0, s, d, j);

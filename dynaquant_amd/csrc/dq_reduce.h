@@ -39,6 +39,51 @@ __device__ __forceinline__ double tile_sum16(const double* p, int ntile, int Bp,
     return s;
 }
 
+// tile_sum16 of two tile columns p and q at once (16 loads in flight per
+// lane), each summed in exactly tile_sum16's order, by NW waves (NW a divisor
+// of 16, every wave of the block); lds: 32 x 64 doubles; one barrier inside;
+// the sums are valid in wave 0
+template <int NW>
+__device__ __forceinline__ void tile_sum16_pair(const double* p, const double* q, int ntile, int Bp,
+                                                double (*lds)[64], double& sp, double& sq)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int w = wave; w < 16; w += NW) {
+        double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double c[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        int t = w;
+        for (; t + 112 < ntile; t += 128) {
+            double x[8], y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                x[u] = p[(size_t)(t + 16 * u) * Bp];
+                y[u] = q[(size_t)(t + 16 * u) * Bp];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                a[u] += x[u];
+                c[u] += y[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (t + 16 * u < ntile) {
+                a[u] += p[(size_t)(t + 16 * u) * Bp];
+                c[u] += q[(size_t)(t + 16 * u) * Bp];
+            }
+        lds[w][lane] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+        lds[16 + w][lane] = ((c[0] + c[1]) + (c[2] + c[3])) + ((c[4] + c[5]) + (c[6] + c[7]));
+    }
+    __syncthreads();
+    sp = 0.0;
+    sq = 0.0;
+    if (wave == 0)
+        for (int v = 0; v < 16; ++v) {
+            sp += lds[v][lane];
+            sq += lds[16 + v][lane];
+        }
+}
+
 // Task `task` of the Gram-row reduction of Lanczos iteration j (j >= 1), run
 // by the leading blocks of the SpMM launch (gram_task_blocks): quantity q of
 // update j-1 (the Gram row <v_l|v_j> as re/im pairs, then |v_j|^2) for one
