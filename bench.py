@@ -12,8 +12,9 @@ steps), one hipGraph launch.  value = B * steps / seconds.
 
 Also reported: the plain FP64 single-trajectory GPU path on the same tube,
 the mixed-vs-FP64 wall-population error over the whole run, the roofline of
-the dominant kernel and of the SpMM from HIP events, and the FP64 CPU oracle
-timed on the host cores.
+the dominant kernel (HIP events handed to each of its launches, which hold
+the kernel's own start and end) and of the SpMM (HIP events recorded around
+each launch), and the FP64 CPU oracle timed on the host cores.
 
 python bench.py [--gpus 1] [--steps 2000] [--warmup 20]
 """
@@ -343,10 +344,13 @@ def main():
                 "frac": upd_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(upd_name),
                 "kernel": upd_name + " (KW = ceil((j+1)/4) = 1..6 for j = 0..23; the NT = max(0, KW - 2)"
                           " oldest groups of 4 vectors read with the nontemporal hint)",
-                "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms}
+                "bytes_per_launch": b_upd, "avg_launch_ms": upd_ms,
+                "timing": "HIP events passed to every update launch of the profiled steps "
+                          "(hipExtLaunchKernelGGL: the kernel's own start and end) on the system's stream"}
     spmm_roof = {"bound": "hbm", "achieved": spmm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": spmm_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(spmm_name),
-                 "kernel": spmm_name, "bytes_per_launch": b_spmm, "avg_launch_ms": spmm_ms}
+                 "kernel": spmm_name, "bytes_per_launch": b_spmm, "avg_launch_ms": spmm_ms,
+                 "timing": "HIP events recorded around every SpMM launch of the profiled steps"}
     roof = spmm_roof if dominant == "spmm" else upd_roof
 
     # ---- plain FP64 single-trajectory path, same tube

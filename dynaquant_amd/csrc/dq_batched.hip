@@ -1101,7 +1101,8 @@ static void launch_update_nt(const Dev& d, int j, int nt, dim3 grid, hipStream_t
             return;
         }
     }
-    hipLaunchKernelGGL((k_update_b<VT, KW, NT>), grid, dim3(kThreads), 0, s, d, j);
+    if (!launch_with_events(k_update_b<VT, KW, NT>, grid, dim3(kThreads), s, d, j))
+        hipLaunchKernelGGL((k_update_b<VT, KW, NT>), grid, dim3(kThreads), 0, s, d, j);
 }
 
 template <typename VT, int KW>

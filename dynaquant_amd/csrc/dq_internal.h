@@ -6,6 +6,7 @@
 //   per-trajectory scalars are [k][Bp];
 //   partial sums are [q][tile][Bp].
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -176,6 +177,26 @@ hipError_t launch_reduce_wide(const Dev& d, int nq, int ntile, int dst, hipStrea
 // (part, 2j+1 quantities) into red[0..2j], in one launch
 hipError_t launch_reduce_all(const Dev& d, int j, hipStream_t s);
 hipError_t launch_update_b(const Dev& d, int j, hipStream_t s);
+
+// Timing events of the launch in progress (per-kernel profiling): a launcher
+// that takes them launches through hipExtLaunchKernelGGL, whose events carry
+// the kernel's own start and end (no dispatch gap), and sets `used`.
+struct KernelEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool used = false;
+};
+extern thread_local KernelEvents g_kev;
+
+// launch `kernel` with the profiling events when there are any; false: the
+// caller launches it the ordinary way
+template <typename F, typename... Args>
+inline bool launch_with_events(F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args)
+{
+    if (!g_kev.start) return false;
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, g_kev.start, g_kev.stop, 0, args...);
+    g_kev.used = true;
+    return true;
+}
 // ring SpMM (dq_ring.hip)
 hipError_t launch_spmm_ring(const Dev& d, int j, int last, hipStream_t s);
 size_t spmm_ring_lds_bytes(int waves, int rows);

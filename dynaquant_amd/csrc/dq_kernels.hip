@@ -596,10 +596,19 @@ template <bool PLAIN, typename VT>
 static void launch_update_t(const Dev& d, int j, hipStream_t s)
 {
     dim3 grid(elem_grid(d, d.ntile_e)), block(kThreads);
-    if (d.m <= 8) hipLaunchKernelGGL((k_update<PLAIN, VT, 8>), grid, block, 0, s, d, j);
-    else if (d.m <= 16) hipLaunchKernelGGL((k_update<PLAIN, VT, 16>), grid, block, 0, s, d, j);
-    else if (d.m <= 24) hipLaunchKernelGGL((k_update<PLAIN, VT, 24>), grid, block, 0, s, d, j);
-    else hipLaunchKernelGGL((k_update<PLAIN, VT, 32>), grid, block, 0, s, d, j);
+    if (d.m <= 8) {
+        if (!launch_with_events(k_update<PLAIN, VT, 8>, grid, block, s, d, j))
+            hipLaunchKernelGGL((k_update<PLAIN, VT, 8>), grid, block, 0, s, d, j);
+    } else if (d.m <= 16) {
+        if (!launch_with_events(k_update<PLAIN, VT, 16>, grid, block, s, d, j))
+            hipLaunchKernelGGL((k_update<PLAIN, VT, 16>), grid, block, 0, s, d, j);
+    } else if (d.m <= 24) {
+        if (!launch_with_events(k_update<PLAIN, VT, 24>, grid, block, s, d, j))
+            hipLaunchKernelGGL((k_update<PLAIN, VT, 24>), grid, block, 0, s, d, j);
+    } else {
+        if (!launch_with_events(k_update<PLAIN, VT, 32>, grid, block, s, d, j))
+            hipLaunchKernelGGL((k_update<PLAIN, VT, 32>), grid, block, 0, s, d, j);
+    }
 }
 
 hipError_t launch_update(const Dev& d, int j, hipStream_t s)
