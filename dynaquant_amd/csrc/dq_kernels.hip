@@ -724,6 +724,15 @@ __device__ void mm_cplx(int n, const double2* A, const double2* B, double2* C)
     __syncthreads();
 }
 
+// lane l's value of x, read by every lane of the wave (l uniform)
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Returns the scaling exponent s (or -1 when s >= 50); result in *Eout.
 __device__ int pade_block(int m, int aug, const double* al, const double* be, double theta,
                           double* sm, const double2** Eout)
@@ -745,8 +754,17 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     if (threadIdx.x < 64) {
         const int i = threadIdx.x;
         double rs = 0.0;
-        if (i < n)
-            for (int jj = 0; jj < n; ++jj) rs += __builtin_fabs(cim * tbar_at(i, jj, m, aug, al, be));
+        // the nonzeros of row i of Tbar in column order (tbar_at's pattern;
+        // the zeros of the full row add +0 to a sum that is never -0, so
+        // this is the full row's sum bit for bit)
+        if (i < m) {
+            if (i >= 1) rs += __builtin_fabs(cim * be[i - 1]);
+            rs += __builtin_fabs(cim * al[i]);
+            if (i + 1 < m) rs += __builtin_fabs(cim * be[i]);
+            if (i == 0 && m < n) rs += __builtin_fabs(cim * 1.0);
+        } else if (i < n) {
+            if (i + 1 < n) rs += __builtin_fabs(cim * 1.0);
+        }
         for (int off = 32; off > 0; off >>= 1) rs = __builtin_fmax(rs, __shfl_xor(rs, off, 64));
         if (threadIdx.x == 0) {
             double nrm = rs;
@@ -799,20 +817,35 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     }
     for (int k = 0; k < n; ++k) {
         if (threadIdx.x < 64) {
+            // wave 0: the pivot stays on the diagonal unless a later row is
+            // strictly larger (one ballot); only then the shuffle reduction
+            // finds the first maximum.  The pivot's reciprocal comes from the
+            // register copy of column k, before any swap.
             const int i = k + threadIdx.x;
             double v = -1.0;
             int idx = n;
+            double2 a = make_double2(0.0, 0.0);
             if (i < n) {
-                const double2 a = X0[i * n + k];
+                a = X0[i * n + k];
                 v = __builtin_fabs(a.x) + __builtin_fabs(a.y);
                 idx = i;
             }
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ov = __shfl_xor(v, off, 64);
-                const int oi = __shfl_xor(idx, off, 64);
-                if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+            int p = k;
+            if (__ballot(v > readlane_f64(v, 0)) != 0) {
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(v, off, 64);
+                    const int oi = __shfl_xor(idx, off, 64);
+                    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+                }
+                p = __builtin_amdgcn_readfirstlane(idx);
             }
-            if (threadIdx.x == 0) s_piv = idx;
+            const double px = readlane_f64(a.x, p - k), py = readlane_f64(a.y, p - k);
+            if (threadIdx.x == 0) {
+                const double den = px * px + py * py;
+                s_inv[0] = px / den;
+                s_inv[1] = -py / den;
+                s_piv = p;
+            }
         }
         __syncthreads();
         const int p = s_piv;
@@ -823,13 +856,6 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
             }
             __syncthreads();
         }
-        if (threadIdx.x == 0) {
-            const double2 pv = X0[k * n + k];
-            const double den = pv.x * pv.x + pv.y * pv.y;
-            s_inv[0] = pv.x / den;
-            s_inv[1] = -pv.y / den;
-        }
-        __syncthreads();
         const double ir = s_inv[0], ii = s_inv[1];
         for (int jj = threadIdx.x; jj < 2 * n; jj += blockDim.x) {
             double2* row = jj < n ? &X0[k * n + jj] : &Bm[k * n + (jj - n)];
