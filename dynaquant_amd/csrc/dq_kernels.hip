@@ -745,9 +745,6 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     double* X0r = sm;
     double* X1r = sm + 2 * kNN;
     __shared__ int s_scale;
-    __shared__ int s_piv;
-    __shared__ double s_inv[2];
-    __shared__ double2 s_f[kNMax];
     const double cim = -theta;            // A = i cim Tbar  (propagate.cu:91-92)
     // infinity norm: row sums of |A_ij| in column order, as the oracle does,
     // one row per lane of wave 0, then the (exact) maximum over the rows
@@ -804,9 +801,19 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
     double2* Bm = reinterpret_cast<double2*>(R0);       // R0, R1 are dead
     for (int e = threadIdx.x; e < nn; e += blockDim.x) Bm[e] = make_double2(0.0, X1r[nn + e]); // Num = i Ur
     __syncthreads();
-    // Gauss-Jordan with partial pivoting (|re|+|im|, first maximum).  Each
-    // thread owns the same elements of [X0 | Bm] at every pivot: their (row,
-    // column) pairs are computed once, outside the pivot loop.
+    // Gauss-Jordan with partial pivoting (|re|+|im|, first maximum) on
+    // [Den | Num], from one pair of LDS buffers (G: Den part, Num part) into
+    // the other (H) at every pivot: the row swap, the scaling of the pivot row
+    // and the elimination are one pass and one barrier.  Every wave finds the
+    // pivot itself from column k in its registers: it stays on the diagonal
+    // unless a later row is strictly larger (one ballot), and only then a
+    // shuffle reduction finds the first maximum; every thread takes the
+    // pivot's reciprocal.  Each thread owns the same elements at every pivot:
+    // their (row, column) pairs are computed once, outside the pivot loop.
+    double2* Ga = X0;
+    double2* Gb = Bm;
+    double2* Ha = X1;
+    double2* Hb = reinterpret_cast<double2*>(sm + 6 * kNN);
     constexpr int kPer = (2 * kNMax * kNMax + kPadeThreads - 1) / kPadeThreads;
     int er[kPer], ec[kPer];
 #pragma unroll
@@ -815,87 +822,64 @@ __device__ int pade_block(int m, int aug, const double* al, const double* be, do
         er[r] = e < n * 2 * n ? e / (2 * n) : -1;
         ec[r] = e < n * 2 * n ? e % (2 * n) : 0;
     }
+    const int lane = threadIdx.x & 63;
     for (int k = 0; k < n; ++k) {
-        if (threadIdx.x < 64) {
-            // wave 0: the pivot stays on the diagonal unless a later row is
-            // strictly larger (one ballot); only then the shuffle reduction
-            // finds the first maximum.  The pivot's reciprocal comes from the
-            // register copy of column k, before any swap.
-            const int i = k + threadIdx.x;
-            double v = -1.0;
-            int idx = n;
-            double2 a = make_double2(0.0, 0.0);
-            if (i < n) {
-                a = X0[i * n + k];
-                v = __builtin_fabs(a.x) + __builtin_fabs(a.y);
-                idx = i;
-            }
-            int p = k;
-            if (__ballot(v > readlane_f64(v, 0)) != 0) {
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ov = __shfl_xor(v, off, 64);
-                    const int oi = __shfl_xor(idx, off, 64);
-                    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-                }
-                p = __builtin_amdgcn_readfirstlane(idx);
-            }
-            const double px = readlane_f64(a.x, p - k), py = readlane_f64(a.y, p - k);
-            if (threadIdx.x == 0) {
-                const double den = px * px + py * py;
-                s_inv[0] = px / den;
-                s_inv[1] = -py / den;
-                s_piv = p;
-            }
+        const int i0 = k + lane;
+        double v = -1.0;
+        int idx = n;
+        double2 a0 = make_double2(0.0, 0.0);
+        if (i0 < n) {
+            a0 = Ga[i0 * n + k];
+            v = __builtin_fabs(a0.x) + __builtin_fabs(a0.y);
+            idx = i0;
         }
-        __syncthreads();
-        const int p = s_piv;
-        if (p != k) {
-            for (int jj = threadIdx.x; jj < n; jj += blockDim.x) {
-                double2 t0 = X0[k * n + jj]; X0[k * n + jj] = X0[p * n + jj]; X0[p * n + jj] = t0;
-                double2 t1 = Bm[k * n + jj]; Bm[k * n + jj] = Bm[p * n + jj]; Bm[p * n + jj] = t1;
+        int p = k;
+        if (__ballot(v > readlane_f64(v, 0)) != 0) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(v, off, 64);
+                const int oi = __shfl_xor(idx, off, 64);
+                if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
             }
-            __syncthreads();
+            p = __builtin_amdgcn_readfirstlane(idx);
         }
-        const double ir = s_inv[0], ii = s_inv[1];
-        for (int jj = threadIdx.x; jj < 2 * n; jj += blockDim.x) {
-            double2* row = jj < n ? &X0[k * n + jj] : &Bm[k * n + (jj - n)];
-            const double2 a = *row;
-            *row = make_double2(a.x * ir - a.y * ii, a.x * ii + a.y * ir);
-        }
-        for (int i = threadIdx.x; i < n; i += blockDim.x) s_f[i] = X0[i * n + k];
-        __syncthreads();
+        const double px = readlane_f64(a0.x, p - k), py = readlane_f64(a0.y, p - k);
+        const double den = px * px + py * py;
+        const double ir = px / den, ii = -py / den;
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const int i = er[r], jj = ec[r];
-            if (i < 0 || i == k) continue;
-            const double2 f = s_f[i];
-            if (jj < n) {
-                if (jj < k) continue;
-                const double2 u = X0[k * n + jj];
-                double2 a = X0[i * n + jj];
-                a.x -= f.x * u.x - f.y * u.y;
-                a.y -= f.x * u.y + f.y * u.x;
-                X0[i * n + jj] = a;
-            } else {
-                const int c = jj - n;
-                const double2 u = Bm[k * n + c];
-                double2 a = Bm[i * n + c];
-                a.x -= f.x * u.x - f.y * u.y;
-                a.y -= f.x * u.y + f.y * u.x;
-                Bm[i * n + c] = a;
+            if (i < 0) continue;
+            const bool num = jj >= n;
+            const int c = num ? jj - n : jj;
+            if (!num && c < k) continue;                // eliminated columns are not read again
+            const double2* G = num ? Gb : Ga;
+            double2* H = num ? Hb : Ha;
+            const double2 x = G[p * n + c];                // pivot row, scaled
+            const double2 u = make_double2(x.x * ir - x.y * ii, x.x * ii + x.y * ir);
+            if (i == k) {
+                H[i * n + c] = u;
+                continue;
             }
+            const int si = i == p ? k : i;                 // row i after the swap of rows k and p
+            const double2 f = Ga[si * n + k];
+            double2 a = G[si * n + c];
+            a.x -= f.x * u.x - f.y * u.y;
+            a.y -= f.x * u.y + f.y * u.x;
+            H[i * n + c] = a;
         }
         __syncthreads();
+        double2* t0 = Ga; Ga = Ha; Ha = t0;
+        double2* t1 = Gb; Gb = Hb; Hb = t1;
     }
     // E = I + 2X
     for (int e = threadIdx.x; e < nn; e += blockDim.x) {
         const int i = e / n, jj = e % n;
-        double2 a = Bm[e];
-        Bm[e] = make_double2(2.0 * a.x + (i == jj ? 1.0 : 0.0), 2.0 * a.y);
+        double2 a = Gb[e];
+        Gb[e] = make_double2(2.0 * a.x + (i == jj ? 1.0 : 0.0), 2.0 * a.y);
     }
     __syncthreads();
-    double2* cur = Bm;
-    double2* oth = X1;
+    double2* cur = Gb;
+    double2* oth = Hb;
     for (int k = 0; k < s; ++k) {
         mm_cplx(n, cur, cur, oth);
         double2* t2 = cur; cur = oth; oth = t2;
@@ -993,10 +977,13 @@ __global__ __launch_bounds__(kPadeThreads) void k_pade(Dev d)
     }
 }
 
-static size_t pade_lds_bytes() { return sizeof(double) * 6 * kNN; }
+// X0, X1, R0/R1 and the second Num buffer of the elimination: 8 kNN doubles
+static size_t pade_lds_bytes() { return sizeof(double) * 8 * kNN; }
+static hipError_t pade_lim();
 
 hipError_t launch_pade(const Dev& d, hipStream_t s)
 {
+    if (hipError_t e = pade_lim(); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pade, dim3(d.sn), dim3(kPadeThreads), pade_lds_bytes(), s, d);
     return hipGetLastError();
 }
@@ -1024,9 +1011,24 @@ __global__ __launch_bounds__(kPadeThreads) void k_pade_tbar(int m, const double*
     }
 }
 
+// lift the dynamic LDS limit of both Pade kernels (once per process)
+static hipError_t pade_lim()
+{
+    static const hipError_t e = [] {
+        hipError_t r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pade),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)pade_lds_bytes());
+        if (r == hipSuccess)
+            r = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_pade_tbar),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)pade_lds_bytes());
+        return r;
+    }();
+    return e;
+}
+
 hipError_t launch_pade_tbar(int m, int count, const double* alpha, const double* beta,
                             double theta, double2* E, int* s_out, hipStream_t s)
 {
+    if (hipError_t e = pade_lim(); e != hipSuccess) return e;
     hipLaunchKernelGGL(k_pade_tbar, dim3(count), dim3(kPadeThreads), pade_lds_bytes(), s, m, alpha, beta,
                        theta, E, s_out);
     return hipGetLastError();
