@@ -2608,37 +2608,41 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
                 graph_choices.push_back({ms_min, t});
             }
         }
-        // the three fastest graphs timed again over 20 replays each: the
-        // first pass separates them by less than its own spread
+        // the three fastest graphs timed again: the first pass separates them
+        // by less than its own spread.  Two rounds over the three (drift
+        // between the candidates cancels), each five batches of ten replays
+        // back to back; a candidate's figure is its fastest batch.
         std::stable_sort(graph_choices.begin(), graph_choices.end(),
                          [](const std::pair<float, dq_tuning>& a, const std::pair<float, dq_tuning>& b) {
                              return a.first < b.first;
                          });
         if (graph_choices.size() > 3) graph_choices.resize(3);
-        double best_graph_ms = 1e300;
-        for (const auto& cand : graph_choices) {
-            if (e != hipSuccess || st != DQ_OK) break;
-            dq_tuning t = cand.second;
-            const int ns = t.concurrent_slices;
-            t.concurrent_slices = 0;
-            if ((st = apply_tuning(sys, t)) != DQ_OK) break;
-            hipGraphExec_t ge = nullptr;
-            if ((e = capture_tuning_slices(sys, ns, &ge)) != hipSuccess) break;
-            float ms_min = 1e30f;
-            e = hipGraphLaunch(ge, sys->stream);                                // warm-up
-            for (int r = 0; r < std::max(reps, 20) && e == hipSuccess; ++r) {
-                float ms = 0.f;
-                if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
-                if ((e = hipGraphLaunch(ge, sys->stream)) != hipSuccess) break;
-                if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
-                if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
-                if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
-                ms_min = std::min(ms_min, ms);
+        std::vector<double> fastest(graph_choices.size(), 1e300);
+        for (int round = 0; round < 2; ++round)
+            for (size_t c = 0; c < graph_choices.size(); ++c) {
+                if (e != hipSuccess || st != DQ_OK) break;
+                dq_tuning t = graph_choices[c].second;
+                const int ns = t.concurrent_slices;
+                t.concurrent_slices = 0;
+                if ((st = apply_tuning(sys, t)) != DQ_OK) break;
+                hipGraphExec_t ge = nullptr;
+                if ((e = capture_tuning_slices(sys, ns, &ge)) != hipSuccess) break;
+                e = hipGraphLaunch(ge, sys->stream);                            // warm-up
+                for (int r = 0; r < 5 && e == hipSuccess; ++r) {
+                    float ms = 0.f;
+                    if ((e = hipEventRecord(sys->ev0, sys->stream)) != hipSuccess) break;
+                    for (int q = 0; q < 10 && e == hipSuccess; ++q) e = hipGraphLaunch(ge, sys->stream);
+                    if (e != hipSuccess) break;
+                    if ((e = hipEventRecord(sys->ev1, sys->stream)) != hipSuccess) break;
+                    if ((e = hipEventSynchronize(sys->ev1)) != hipSuccess) break;
+                    if ((e = hipEventElapsedTime(&ms, sys->ev0, sys->ev1)) != hipSuccess) break;
+                    fastest[c] = std::min(fastest[c], (double)ms / 10.0);
+                }
+                hipGraphExecDestroy(ge);
             }
-            hipGraphExecDestroy(ge);
-            if (e != hipSuccess) break;
-            if (ms_min < best_graph_ms) { best_graph_ms = ms_min; best = cand.second; }
-        }
+        double best_graph_ms = 1e300;
+        for (size_t c = 0; c < graph_choices.size(); ++c)
+            if (fastest[c] < best_graph_ms) { best_graph_ms = fastest[c]; best = graph_choices[c].second; }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(d.psi, saved, sizeof(double2) * nB, hipMemcpyDeviceToDevice, sys->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d.de, saved_de, sizeof(double) * nB, hipMemcpyDeviceToDevice, sys->stream);

@@ -672,8 +672,9 @@ __device__ __forceinline__ double tbar_at(int i, int jj, int m, int aug, const d
 }
 
 // Products of n x n matrices in LDS, a 2 x 2 block of C per thread (one pass
-// for n <= 32: half the LDS reads of one element per thread).  Each element
-// is the same k-ordered sum as a one-element-per-thread product.
+// for n <= 32: half the LDS reads of one element per thread), the k loop
+// unrolled four times so that the loads of four k are in flight together.
+// Each element is the same k-ordered sum as a one-element-per-thread product.
 __device__ void mm_real(int n, const double* A, const double* B, double* C)
 {
     const int nb = (n + 1) >> 1;
@@ -681,6 +682,7 @@ __device__ void mm_real(int n, const double* A, const double* B, double* C)
         const int i0 = 2 * (blk / nb), j0 = 2 * (blk % nb);
         const int i1 = i0 + 1 < n ? i0 + 1 : i0, j1 = j0 + 1 < n ? j0 + 1 : j0;
         double s00 = 0.0, s01 = 0.0, s10 = 0.0, s11 = 0.0;
+#pragma unroll 4
         for (int k = 0; k < n; ++k) {
             const double a0 = A[i0 * n + k], a1 = A[i1 * n + k];
             const double b0 = B[k * n + j0], b1 = B[k * n + j1];
@@ -704,6 +706,7 @@ __device__ void mm_cplx(int n, const double2* A, const double2* B, double2* C)
         const int i0 = 2 * (blk / nb), j0 = 2 * (blk % nb);
         const int i1 = i0 + 1 < n ? i0 + 1 : i0, j1 = j0 + 1 < n ? j0 + 1 : j0;
         double r00 = 0.0, q00 = 0.0, r01 = 0.0, q01 = 0.0, r10 = 0.0, q10 = 0.0, r11 = 0.0, q11 = 0.0;
+#pragma unroll 4
         for (int k = 0; k < n; ++k) {
             const double2 a0 = A[i0 * n + k], a1 = A[i1 * n + k];
             const double2 b0 = B[k * n + j0], b1 = B[k * n + j1];
