@@ -2496,6 +2496,14 @@ dq_status dq_autotune(dq_system* sys, int32_t reps, const char* cache_path, dq_t
     }
     Dev& d = sys->d;
     sys->drop_graph();
+    // the trial launches stay out of the kernel timings, and the captured
+    // trial graphs hold no profiling events
+    struct ProfilingOff {
+        dq_system* s;
+        bool was;
+        ~ProfilingOff() { s->profiling = was; }
+    } profiling_off{sys, sys->profiling};
+    sys->profiling = false;
     HIP_TRY(hipStreamSynchronize(sys->stream));
     const size_t nB = (size_t)d.n * d.Bp;
     // saved state: psi, then the noise offsets and the diagonal

@@ -199,3 +199,22 @@ def test_matrix_core_spmm_refused_beyond_32bit_offsets(engine):
         assert s.tuning()["spmm_rows_per_wave"] == 8
     finally:
         s.close()
+
+
+def test_autotune_under_profiling(engine):
+    """autotune with per-kernel profiling on: its trial launches stay out of
+    the kernel timings, and a profiled run afterwards times every update
+    launch with the events the launch itself takes"""
+    sysd = ring_tube(engine)
+    n = sysd["n"]
+    psi0 = np.zeros(n, complex); psi0[0] = 1
+    s = make_engine(engine, sysd, 16, 1.0, B=128, force_batched=True, precision=engine.MIXED)
+    s.set_state(psi0)
+    s.set_profiling(True)
+    s.autotune(reps=1)
+    assert all(v[1] == 0 for v in s.kernel_times().values())
+    s.run(3)
+    kt = s.kernel_times()
+    assert kt["update"][1] == 3 * 16 and kt["update"][0] > 0
+    assert kt["spmm"][1] == 3 * 16 and kt["pade"][1] == 3
+    s.close()
